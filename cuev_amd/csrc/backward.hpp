@@ -1,0 +1,12 @@
+// backward.hpp — eigendecomposition backward pass.
+#pragma once
+
+#include "common.hpp"
+
+namespace cuev {
+
+template <typename T>
+int eig_backward(const T *V, const T *lam, const T *Vbar, const T *lambar, int n, T *Abar,
+                 double gap_tol, int *n_masked, hipStream_t s);
+
+} // namespace cuev

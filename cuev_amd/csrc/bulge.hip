@@ -1,0 +1,230 @@
+// bulge.hip — band -> tridiagonal reduction (bulge chasing) and (d, e) extraction.
+//
+// Algorithm (reference: src/cuda/bulge.cu:64-224, Algorithm 2 of Wang et al.):
+// sweep s = 0..n-3 eliminates column s below its subdiagonal with a Householder
+// reflector on rows [s+1, s+b], then chases the bulge this creates down the band:
+// hop h >= 1 eliminates the bulge column cq = s+1+(h-1)b on rows
+// [cq+b, cq+2b-1]. Each hop is the similarity A <- H A H with H = I - 2 w w^T,
+// ||w|| = 1, applied as A <- A - w y^T - y w^T, p = 2 A w, kappa = w^T p,
+// y = p - kappa w. The reflector of hop h of sweep s occupies rows
+// [s+1+hb, min(s+(h+1)b, n-1)] and is stored for the back-transform.
+//
+// Our design (not the reference's):
+//  * Reflector convention: if the vector below the pivot is exactly zero the
+//    hop is the identity (w = 0), as LAPACK's dlarfg does; the reference's
+//    normalised reflector would flip the sign of the pivot instead.
+//  * Each hop works on a copy of its window in LDS: the packed band columns
+//    [cq, r1] (all 2b stored rows), which hold every entry the hop reads or
+//    writes. The Householder vector is formed by wave 0 alone (L <= 64 = one
+//    wave64), with FP64 norms.
+//  * Dependency: hop (s, cq, r1) touches packed columns [cq, r1] only, and the
+//    previous sweep's hops touch columns [cq'', r1''] with cq'' increasing, so
+//    sweep s may start the hop once sweep s-1 has completed its hop with
+//    cq'' = r1 (progress flag prog[s-1] >= r1). That is two hops of lag instead
+//    of the reference's margin of 3b columns (bulge.cu:193-199), which shortens
+//    the ~ (lag * n) critical path.
+//  * Flags are agent-scope acquire/release atomics, every wait is bounded, and
+//    the grid never exceeds the number of co-resident workgroups, so the lowest
+//    unfinished sweep can always proceed.
+//  * U is stored shifted: U[(r - s - 1) + s*ldu] holds row r of sweep s's
+//    reflectors, so the back-transform reads aligned, contiguous segments.
+#include "bulge.hpp"
+
+#include <algorithm>
+
+#include "util.hpp"
+
+namespace cuev {
+namespace {
+
+constexpr int kBCThreads = 256;
+constexpr int kDone = 0x3fffffff; // prog value of a finished sweep
+
+template <typename T>
+__global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 ldu, int n, int b,
+                                                              int *prog, int *err) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int ldb = 2 * b;       // packed band leading dimension
+    const int WLD = 2 * b + 2;   // LDS window leading dimension (padded against bank conflicts)
+    T *win = reinterpret_cast<T *>(smem_raw);          // [2b cols][WLD]
+    T *sw = win + (size_t)2 * b * WLD;                 // [b]   reflector w
+    T *sp = sw + b;                                    // [3b]  p, then y
+    __shared__ double s_beta;
+    __shared__ double s_kappa;
+    __shared__ int s_go;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    for (int s = blockIdx.x; s < n - 2; s += gridDim.x) {
+        for (int h = 0;; ++h) {
+            const int cq = (h == 0) ? s : s + 1 + (h - 1) * b;
+            const int r0 = (h == 0) ? s + 1 : cq + b;
+            if (r0 > n - 1) break;
+            const int r1 = min(r0 + b - 1, n - 1);
+            const int L = r1 - r0 + 1;
+            if (L < 2) {
+                // nothing below the pivot: identity reflector (record it for the back-transform)
+                if (tid == 0) U[(i64)(r0 - s - 1) + (i64)s * ldu] = T(0);
+                break;
+            }
+            const int hi = min(n - 1, r1 + b);  // last row touched
+            const int ncol = r1 - cq + 1;       // window columns [cq, r1]
+
+            // ---- wait for sweep s-1 to be done with columns <= r1 (acquire)
+            if (s > 0) {
+                if (tid == 0) {
+                    long long it = 0;
+                    int go = 1;
+                    while (flag_load_acquire(prog + (s - 1)) < r1) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++it > kSpinLimit) {
+                            __hip_atomic_store(err, (int)DEV_WATCHDOG, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            go = 0;
+                            break;
+                        }
+                    }
+                    s_go = go;
+                }
+                __syncthreads();
+                if (!s_go) return;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+
+            // ---- load the window (packed columns cq..r1, rows 0..2b-1)
+            const T *gsrc = Bp + (i64)cq * ldb;
+            for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
+                const int jj = idx / ldb, d = idx - jj * ldb;
+                win[jj * WLD + d] = gsrc[idx];
+            }
+            __syncthreads();
+
+            // ---- Householder vector on x = A[r0:r1, cq] (wave 0)
+            if (wave == 0) {
+                const double x = (lane < L) ? (double)win[(r0 - cq) + lane] : 0.0;
+                const double alpha = __shfl(x, 0, 64);
+                const double tail = wave_sum((lane >= 1) ? x * x : 0.0);
+                double wv = 0.0, beta = alpha;
+                if (tail != 0.0) {
+                    const double nrm = sqrt(alpha * alpha + tail);
+                    beta = (alpha >= 0.0) ? -nrm : nrm;
+                    const double v0 = alpha - beta;
+                    const double inv = 1.0 / sqrt(v0 * v0 + tail);
+                    wv = ((lane == 0) ? v0 : x) * inv;
+                }
+                if (lane < L) sw[lane] = (T)wv;
+                if (lane == 0) s_beta = beta;
+            }
+            __syncthreads();
+
+            // ---- p_i = 2 sum_k A(i, r0+k) w_k for i in [cq, hi]
+            const int nrow = hi - cq + 1;
+            for (int t = tid; t < nrow; t += kBCThreads) {
+                const int i = cq + t;
+                T acc = T(0);
+                for (int q = 0; q < L; ++q) {
+                    const int k = r0 + q;
+                    const T a = (i >= k) ? win[(k - cq) * WLD + (i - k)] : win[(i - cq) * WLD + (k - i)];
+                    acc += a * sw[q];
+                }
+                sp[t] = T(2) * acc;
+            }
+            __syncthreads();
+
+            // ---- kappa = w^T p ; y = p - kappa w
+            if (wave == 0) {
+                const double v = (lane < L) ? (double)sw[lane] * (double)sp[(r0 - cq) + lane] : 0.0;
+                const double kap = wave_sum(v);
+                if (lane == 0) s_kappa = kap;
+            }
+            __syncthreads();
+            {
+                const T kap = (T)s_kappa;
+                for (int t = tid; t < L; t += kBCThreads) sp[(r0 - cq) + t] -= kap * sw[t];
+            }
+            __syncthreads();
+
+            // ---- rank-2 update A(i,j) -= w_i y_j + y_i w_j on the window (i >= j, i <= hi)
+            for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
+                const int jj = idx / ldb, d = idx - jj * ldb;
+                const int j = cq + jj, i = j + d;
+                if (i > hi) continue;
+                const bool wi_nz = (i >= r0 && i <= r1), wj_nz = (j >= r0);
+                if (!wi_nz && !wj_nz) continue;
+                T *a = &win[jj * WLD + d];
+                if (jj == 0) {
+                    // eliminated column: exact result H x = beta e_0
+                    *a = (i == r0) ? (T)s_beta : T(0);
+                    continue;
+                }
+                const T wi = wi_nz ? sw[i - r0] : T(0);
+                const T wj = wj_nz ? sw[j - r0] : T(0);
+                *a -= wi * sp[j - cq] + sp[i - cq] * wj;
+            }
+            __syncthreads();
+
+            // ---- write back the window and the reflector
+            T *gdst = Bp + (i64)cq * ldb;
+            for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
+                const int jj = idx / ldb, d = idx - jj * ldb;
+                gdst[idx] = win[jj * WLD + d];
+            }
+            T *ucol = U + (i64)(r0 - s - 1) + (i64)s * ldu;
+            for (int t = tid; t < L; t += kBCThreads) ucol[t] = sw[t];
+
+            // ---- publish progress (release)
+            __threadfence();
+            __syncthreads();
+            if (tid == 0) flag_store_release(prog + s, cq);
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) flag_store_release(prog + s, kDone);
+    }
+}
+
+template <typename T>
+__global__ void bc_extract_kernel(const T *Bp, T *d, T *e, int n, int b) {
+    const i64 ldb = 2 * b;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = Bp[(i64)i * ldb];
+    if (i < n - 1) e[i] = Bp[1 + (i64)i * ldb];
+    if (i == n - 1) e[i] = T(0);
+}
+
+template <typename T> size_t bc_lds_bytes(int b) {
+    return sizeof(T) * ((size_t)2 * b * (2 * b + 2) + b + 3 * b);
+}
+
+} // namespace
+
+template <typename T> int bc_chase(Workspace<T> &ws, T *Bp, T *d, T *e) {
+    const int n = ws.n, b = ws.b;
+    hipStream_t s = ws.stream;
+    const int nsweeps = n - 2;
+    if (nsweeps > 0) {
+        CUEV_TRY(fill_int(s, ws.prog, n, -1));
+        const size_t lds = bc_lds_bytes<T>(b);
+        CUEV_HIP(hipFuncSetAttribute((const void *)bc_chase_kernel<T>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int per_cu = 0;
+        CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc_chase_kernel<T>,
+                                                              kBCThreads, lds));
+        const int resident = std::max(1, per_cu * device_cu_count());
+        // at most ~n/(2b) sweeps can be active at once; more workgroups would only spin
+        const int useful = (int)ceil_div(n, 2 * b) + 2;
+        const int grid = std::max(1, std::min({nsweeps, resident, useful}));
+        hipLaunchKernelGGL(bc_chase_kernel<T>, dim3(grid), dim3(kBCThreads), lds, s, Bp, ws.U,
+                           ws.ldu, n, b, ws.prog, ws.err);
+        CUEV_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(bc_extract_kernel<T>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, Bp,
+                       d, e, n, b);
+    CUEV_LAUNCH_CHECK();
+    return CUEV_STATUS_SUCCESS;
+}
+
+template int bc_chase<float>(Workspace<float> &, float *, float *, float *);
+template int bc_chase<double>(Workspace<double> &, double *, double *, double *);
+
+} // namespace cuev

@@ -1,0 +1,31 @@
+// context.hpp — a workspace plus the host staging of the tridiagonal stage.
+#pragma once
+
+#include "workspace.hpp"
+
+namespace cuev {
+
+template <typename T> struct Context {
+    Workspace<T> ws;
+    // pinned host staging for the tridiagonal stage
+    T *h_d = nullptr;     // n
+    T *h_e = nullptr;     // n
+    T *h_z = nullptr;     // n x n (Q_d)
+    // host LAPACK workspace (pageable)
+    T *h_work = nullptr;
+    int *h_iwork = nullptr;
+    i64 lwork = 0;
+    int liwork = 0;
+    hipEvent_t ev[12] = {};
+    int dtype = 0;
+};
+
+template <typename T> int context_create(Context<T> *&ctx, int n, int b, int k, hipStream_t s);
+template <typename T> void context_destroy(Context<T> *ctx);
+
+// Process-wide cache used by the drop-in entry points: at most one idle context
+// per (device, precision); a busy or mismatched entry is bypassed.
+template <typename T> int context_acquire(Context<T> *&ctx, int n, int b, int k, hipStream_t s);
+template <typename T> void context_release(Context<T> *ctx);
+
+} // namespace cuev

@@ -1,0 +1,419 @@
+// gemm.hip — MFMA tile engine for gfx950 (FP64 and FP32).
+//
+// Block tile BM x BN x 16, 256 threads = 4 wave64s arranged 2 x 2, each wave
+// owning a (BM/2) x (BN/2) sub-tile of 16 x 16 MFMA fragments. Operands are
+// staged global -> registers -> LDS (two LDS buffers, one barrier per k-step,
+// the next k-tile's global loads in flight while the current one computes).
+//
+// The MFMA is issued with the B fragment in the instruction's A slot and the A
+// fragment in its B slot, so the accumulator's 16 consecutive lanes hold 16
+// consecutive rows of one C column: epilogue stores are coalesced for
+// column-major C. Output lane/register layout (measured on gfx950):
+//   f64 16x16x4: lane l, reg r -> (row l/16 + 4r, col l%16)
+//   f32 16x16x4: lane l, reg r -> (row 4(l/16) + r, col l%16)
+// (row/col of the instruction's D; with the swapped operands D's row is C's
+// column index and D's column is C's row index.)
+//
+// Work distribution: blocks are dispatched round-robin over the 8 XCDs, so the
+// block index is first remapped so that each XCD receives a contiguous range of
+// tiles, then grouped 8 tile-rows at a time, keeping operand panels shared by
+// neighbouring tiles inside one XCD's L2.
+//
+// Tall-skinny products (the DBBR panel products with K = n - j - b, the
+// sbr_back Y^T M) split K over blockIdx.y into a scratch buffer, summed in a
+// fixed order by a second kernel, so results are deterministic.
+#include "gemm.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace cuev {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int BK = 16;
+constexpr int kGroupM = 8;
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int out_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T> struct KParams {
+    const T *a0, *a1;
+    i64 lda0, lda1, asplit;
+    int atrans, asym;
+    const T *b0, *b1;
+    i64 ldb0, ldb1, bsplit;
+    int btrans;
+    T *C;
+    i64 ldc;
+    T *part; // split-K partials: part[s*M*N + m + n*M]
+    i64 M, N, K, kchunk;
+    T alpha, beta;
+    int lower;
+    int tiles_m, tiles_n;
+};
+
+// Map the (remapped) linear tile id to (tile_m, tile_n).
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int lower, int &tm,
+                                            int &tn) {
+    if (lower) {
+        // enumerate pairs tm >= tn: t = tm (tm + 1) / 2 + tn
+        int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((i64)(r + 1) * (r + 2) / 2 <= t) ++r;
+        while ((i64)r * (r + 1) / 2 > t) --r;
+        tm = r;
+        tn = t - r * (r + 1) / 2;
+    } else {
+        const int group = kGroupM * tiles_n;
+        const int gid = t / group;
+        const int first_m = gid * kGroupM;
+        const int gsz = min(tiles_m - first_m, kGroupM);
+        const int w = t - gid * group;
+        tm = first_m + w % gsz;
+        tn = w / gsz;
+    }
+}
+
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void gemm_tile_kernel(const KParams<T> p) {
+    constexpr int TM = BM / 2, TN = BN / 2;
+    constexpr int FM = TM / 16, FN = TN / 16;
+    constexpr int NA = BM * BK / kThreads; // A elements per thread per k-tile
+    constexpr int NB = BN * BK / kThreads;
+    constexpr int PAD = 1;
+    using V4 = typename Mfma<T>::v4;
+
+    __shared__ T sA[2][BK][BM + PAD];
+    __shared__ T sB[2][BK][BN + PAD];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    // XCD-contiguous remap of the block index
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, q = nb >> 3, rmd = nb & 7;
+    const int t = xcd * q + min(xcd, rmd) + (bid >> 3);
+    int tm, tn;
+    tile_coords(t, p.tiles_m, p.tiles_n, p.lower, tm, tn);
+    const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
+
+    const i64 kbeg = (i64)blockIdx.y * p.kchunk;
+    const i64 kend = min(p.K, kbeg + p.kchunk);
+    const int nkt = (int)((kend - kbeg + BK - 1) / BK);
+
+    V4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+    T ra[NA], rb[NB];
+    int patA = 0; // 0: m-contiguous, 1: k-contiguous, 2: m-contiguous with symmetric select
+
+    auto load_tiles = [&](i64 k0) {
+        // ---- A tile: rows [m0, m0+BM), k in [k0, k0+BK)
+        const T *pa;
+        i64 lda, ko;
+        if (p.asym) {
+            pa = p.a0;
+            lda = p.lda0;
+            ko = 0;
+            if (m0 >= k0 + BK - 1) patA = 0;
+            else if (m0 + BM - 1 <= k0) patA = 1;
+            else patA = 2;
+        } else {
+            const bool second = p.asplit >= 0 && k0 >= p.asplit;
+            pa = second ? p.a1 : p.a0;
+            lda = second ? p.lda1 : p.lda0;
+            ko = second ? p.asplit : 0;
+            patA = p.atrans ? 1 : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            int ml, kk;
+            if (patA == 1) {
+                ml = tid / BK + (kThreads / BK) * i;
+                kk = tid % BK;
+            } else {
+                ml = tid % BM;
+                kk = tid / BM + (kThreads / BM) * i;
+            }
+            const i64 m = m0 + ml, k = k0 + kk;
+            T v = T(0);
+            if (m < p.M && k < kend) {
+                if (patA == 0) v = pa[m + (k - ko) * lda];
+                else if (patA == 1) v = pa[(k - ko) + m * lda];
+                else v = (m >= k) ? pa[m + k * lda] : pa[k + m * lda];
+            }
+            ra[i] = v;
+        }
+        // ---- B tile: k in [k0, k0+BK), cols [n0, n0+BN)
+        {
+            const bool second = p.bsplit >= 0 && k0 >= p.bsplit;
+            const T *pb = second ? p.b1 : p.b0;
+            const i64 ldb = second ? p.ldb1 : p.ldb0;
+            const i64 ko2 = second ? p.bsplit : 0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                int nl, kk;
+                if (p.btrans) {
+                    nl = tid % BN;
+                    kk = tid / BN + (kThreads / BN) * i;
+                } else {
+                    nl = tid / BK + (kThreads / BK) * i;
+                    kk = tid % BK;
+                }
+                const i64 n = n0 + nl, k = k0 + kk;
+                T v = T(0);
+                if (n < p.N && k < kend)
+                    v = p.btrans ? pb[n + (k - ko2) * ldb] : pb[(k - ko2) + n * ldb];
+                rb[i] = v;
+            }
+        }
+    };
+    auto store_tiles = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            int ml, kk;
+            if (patA == 1) {
+                ml = tid / BK + (kThreads / BK) * i;
+                kk = tid % BK;
+            } else {
+                ml = tid % BM;
+                kk = tid / BM + (kThreads / BM) * i;
+            }
+            sA[buf][kk][ml] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            int nl, kk;
+            if (p.btrans) {
+                nl = tid % BN;
+                kk = tid / BN + (kThreads / BN) * i;
+            } else {
+                nl = tid / BK + (kThreads / BK) * i;
+                kk = tid % BK;
+            }
+            sB[buf][kk][nl] = rb[i];
+        }
+    };
+
+    if (nkt > 0) {
+        load_tiles(kbeg);
+        store_tiles(0);
+    }
+    __syncthreads();
+
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nkt;
+        if (more) load_tiles(kbeg + (i64)(kt + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kr = ks * 4 + (lane >> 4);
+            T af[FM], bf[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = sA[cur][kr][wm * TM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = sB[cur][kr][wn * TN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(bf[j], af[i], acc[i][j]);
+        }
+        if (more) store_tiles(cur ^ 1);
+        __syncthreads();
+    }
+
+    // ---- epilogue
+    const bool partial = p.part != nullptr;
+    T *part = partial ? p.part + (i64)blockIdx.y * p.M * p.N : nullptr;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const i64 m = m0 + wm * TM + i * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn * TN + j * 16 + Mfma<T>::out_row(lane, r);
+                if (m >= p.M || n >= p.N) continue;
+                if (p.lower && m < n) continue;
+                const T v = acc[i][j][r];
+                if (partial) {
+                    part[m + n * p.M] = v;
+                } else {
+                    T *c = p.C + m + n * p.ldc;
+                    *c = (p.beta == T(0)) ? p.alpha * v : p.alpha * v + p.beta * (*c);
+                }
+            }
+        }
+    }
+}
+
+// C = alpha * sum_s part[s] + beta * C (fixed summation order).
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int splits, i64 M,
+                                                            i64 N, T alpha, T beta, T *C, i64 ldc,
+                                                            int lower) {
+    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * N) return;
+    const i64 m = idx % M, n = idx / M;
+    if (lower && m < n) return;
+    T s = T(0);
+    for (int k = 0; k < splits; ++k) s += part[(i64)k * M * N + idx];
+    T *c = C + m + n * ldc;
+    *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
+}
+
+template <typename T, int BM, int BN>
+int launch_cfg(hipStream_t s, KParams<T> &p, int splits, int ntiles) {
+    dim3 grid((unsigned)ntiles, (unsigned)splits);
+    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN>), grid, dim3(kThreads), 0, s, p);
+    CUEV_LAUNCH_CHECK();
+    return CUEV_STATUS_SUCCESS;
+}
+
+} // namespace
+
+size_t gemm_scratch_bytes(i64 M, i64 N, size_t elem) {
+    // split-K is only used when the tile count is below ~one wave of blocks, so
+    // splits * M * N stays below ~3 * (2 * CUs) * 128 * 128 elements.
+    (void)M;
+    (void)N;
+    return (size_t)3 * 512 * 128 * 128 * elem;
+}
+
+template <typename T>
+int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
+            const Operand<T> &B, T beta, T *C, i64 ldc, int lower, Scratch scratch) {
+    if (M < 0 || N < 0 || K < 0)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: negative size M=%lld N=%lld K=%lld",
+                  (long long)M, (long long)N, (long long)K);
+    if (M == 0 || N == 0) return CUEV_STATUS_SUCCESS;
+    if (lower && M != N) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: lower mode needs M == N");
+    if ((A.split >= 0 && A.split % BK) || (B.split >= 0 && B.split % BK))
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: K split must be a multiple of %d", BK);
+    if (ldc < M) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: ldc < M");
+
+    KParams<T> p{};
+    p.a0 = A.p0;
+    p.a1 = A.p1;
+    p.lda0 = A.ld0;
+    p.lda1 = A.ld1;
+    p.asplit = A.split;
+    p.atrans = A.trans;
+    p.asym = A.sym;
+    p.b0 = B.p0;
+    p.b1 = B.p1;
+    p.ldb0 = B.ld0;
+    p.ldb1 = B.ld1;
+    p.bsplit = B.split;
+    p.btrans = B.trans;
+    p.C = C;
+    p.ldc = ldc;
+    p.M = M;
+    p.N = N;
+    p.K = K;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.lower = lower;
+
+    // tile shape
+    int BMs = (M > 64) ? 128 : 64;
+    int BNs = (N > 64) ? 128 : 64;
+    if (lower) BMs = BNs = (N >= 1024) ? 128 : 64;
+    const int cus = device_cu_count();
+    auto count_tiles = [&](int bm, int bn) -> i64 {
+        const i64 tm = ceil_div(M, bm), tn = ceil_div(N, bn);
+        return lower ? tm * (tm + 1) / 2 : tm * tn;
+    };
+    if (!lower && BMs == 128 && BNs == 128 && count_tiles(128, 128) < cus) {
+        // few large tiles: trade tile size for parallelism
+        if (count_tiles(64, 128) >= cus || M < N) BMs = 64;
+        else BNs = 64;
+    }
+    const i64 ntiles = count_tiles(BMs, BNs);
+    if (ntiles > 0x7fffffffLL) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: too many tiles");
+    p.tiles_m = (int)ceil_div(M, BMs);
+    p.tiles_n = (int)ceil_div(N, BNs);
+
+    // split-K for products with few output tiles and a long K
+    int splits = 1;
+    const i64 target = 2LL * cus;
+    if (K == 0) {
+        splits = 0;
+    } else if (ntiles < target / 2 && K >= 1024) {
+        i64 want = std::min<i64>(ceil_div(target, ntiles), K / 512);
+        const i64 cap = (i64)(scratch.bytes / ((size_t)M * N * sizeof(T)));
+        want = std::min(want, cap);
+        if (want > 1) splits = (int)want;
+    }
+    if (splits <= 1 && K > 0) {
+        p.kchunk = round_up(K, BK);
+        p.part = nullptr;
+        splits = 1;
+    } else if (splits > 1) {
+        p.kchunk = round_up(ceil_div(K, splits), BK);
+        splits = (int)ceil_div(K, p.kchunk);
+        p.part = static_cast<T *>(scratch.ptr);
+    }
+
+    if (splits >= 1) {
+        int st;
+        if (BMs == 128 && BNs == 128) st = launch_cfg<T, 128, 128>(s, p, splits, (int)ntiles);
+        else if (BMs == 128) st = launch_cfg<T, 128, 64>(s, p, splits, (int)ntiles);
+        else if (BNs == 128) st = launch_cfg<T, 64, 128>(s, p, splits, (int)ntiles);
+        else st = launch_cfg<T, 64, 64>(s, p, splits, (int)ntiles);
+        if (st) return st;
+    }
+    if (p.part != nullptr || splits == 0) {
+        const i64 total = M * N;
+        const int nsplit = (splits == 0) ? 0 : splits;
+        hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)ceil_div(total, 256)),
+                           dim3(256), 0, s, static_cast<const T *>(p.part), nsplit, M, N, alpha,
+                           beta, C, ldc, lower);
+        CUEV_LAUNCH_CHECK();
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T>
+int syr2k_lower(hipStream_t s, i64 n, i64 k, T alpha, const T *X, i64 ldx, const T *Y, i64 ldy,
+                T beta, T *C, i64 ldc, Scratch scratch) {
+    if (k % BK == 0) {
+        // [X | Y] * [Y | X]^T as one product with K = 2k
+        Operand<T> a = opnd_cat(X, ldx, Y, ldy, k, 0);
+        Operand<T> b = opnd_cat(Y, ldy, X, ldx, k, 1);
+        return gemm_ex<T>(s, n, n, 2 * k, alpha, a, b, beta, C, ldc, 1, scratch);
+    }
+    CUEV_TRY(gemm_ex<T>(s, n, n, k, alpha, opnd(X, ldx, 0), opnd(Y, ldy, 1), beta, C, ldc, 1,
+                        scratch));
+    return gemm_ex<T>(s, n, n, k, alpha, opnd(Y, ldy, 0), opnd(X, ldx, 1), T(1), C, ldc, 1,
+                      scratch);
+}
+
+template int gemm_ex<float>(hipStream_t, i64, i64, i64, float, const Operand<float> &,
+                            const Operand<float> &, float, float *, i64, int, Scratch);
+template int gemm_ex<double>(hipStream_t, i64, i64, i64, double, const Operand<double> &,
+                             const Operand<double> &, double, double *, i64, int, Scratch);
+template int syr2k_lower<float>(hipStream_t, i64, i64, float, const float *, i64, const float *,
+                                i64, float, float *, i64, Scratch);
+template int syr2k_lower<double>(hipStream_t, i64, i64, double, const double *, i64,
+                                 const double *, i64, double, double *, i64, Scratch);
+
+} // namespace cuev

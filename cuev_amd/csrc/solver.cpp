@@ -1,0 +1,181 @@
+// solver.cpp — orchestration of the two-stage eigensolver.
+//
+// Stage order follows the reference (src/cuda/solver.cu:44-57):
+// DBBR -> bulge chasing -> tridiagonal D&C -> back-transform, all GPU work on
+// the caller's stream and the call blocking until the results are in place.
+//
+// Two back-transform orders:
+//  * direct (the reference's, backtransform.cu:232-275): after the host D&C,
+//    evec <- Q_s (Q_b Q_d) with bc_back then sbr_back on Q_d;
+//  * reordered (listed as future work in the reference README:35): while the
+//    host runs dstedc, the GPU builds X = Q_s Q_b (bc_back and sbr_back applied
+//    to the identity); after the D&C only one GEMM evec = X Q_d remains.
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "backtransform.hpp"
+#include "bulge.hpp"
+#include "dbbr.hpp"
+#include "gemm.hpp"
+#include "lapack.hpp"
+#include "util.hpp"
+
+namespace cuev {
+
+void options_defaults(cuev_options &o) {
+    o.bandwidth = 0;
+    o.block = 0;
+    o.precision = CUEV_PRECISION_NATIVE;
+    o.refine_sweeps = 0;
+    o.bt_mode = CUEV_BT_AUTO;
+    o.refine_tol = 0.0;
+}
+
+namespace {
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+} // namespace
+
+template <typename T>
+int tridi_dc(Context<T> &ctx, T *d, T *e, T *eval, T *evec, i64 ldv, double *dc_ms) {
+    Workspace<T> &ws = ctx.ws;
+    const int n = ws.n;
+    hipStream_t s = ws.stream;
+    CUEV_HIP(hipMemcpyAsync(ctx.h_d, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipMemcpyAsync(ctx.h_e, e, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite((double)ctx.h_d[i]) || (i < n - 1 && !std::isfinite((double)ctx.h_e[i])))
+            CUEV_FAIL(CUEV_STATUS_NOT_FINITE, "tridiagonal matrix has a non-finite entry at %d", i);
+    auto t0 = clk::now();
+    CUEV_TRY(stedc<T>(n, ctx.h_d, ctx.h_e, ctx.h_z, n, ctx.h_work, ctx.lwork, ctx.h_iwork,
+                      ctx.liwork));
+    if (dc_ms) *dc_ms = ms_since(t0, clk::now());
+    CUEV_HIP(hipMemcpyAsync(eval, ctx.h_d, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    CUEV_HIP(hipMemcpy2DAsync(evec, sizeof(T) * ldv, ctx.h_z, sizeof(T) * n, sizeof(T) * n, n,
+                              hipMemcpyHostToDevice, s));
+    return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T>
+int back_transform(Context<T> &ctx, const T *Y, const T *W, const T *U, T *evec,
+                   cuev_solve_timer *timer) {
+    Workspace<T> &ws = ctx.ws;
+    hipStream_t s = ws.stream;
+    if (timer) CUEV_HIP(hipEventRecord(ctx.ev[4], s));
+    CUEV_TRY(bc_back<T>(ws, U, evec, ws.ld));
+    if (timer) CUEV_HIP(hipEventRecord(ctx.ev[5], s));
+    CUEV_TRY(sbr_back<T>(ws, Y, W, evec, ws.ld));
+    if (timer) {
+        CUEV_HIP(hipEventRecord(ctx.ev[6], s));
+        CUEV_HIP(hipEventSynchronize(ctx.ev[6]));
+        timer->bt_qb_ms = ev_ms(ctx.ev[4], ctx.ev[5]);
+        timer->bt_qs_ms = ev_ms(ctx.ev[5], ctx.ev[6]);
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T>
+int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode) {
+    Workspace<T> &ws = ctx.ws;
+    const int n = ws.n;
+    hipStream_t s = ws.stream;
+    hipEvent_t *ev = ctx.ev;
+    const auto wall0 = clk::now();
+    if (bt_mode == CUEV_BT_AUTO) bt_mode = (n >= 2048) ? CUEV_BT_REORDERED : CUEV_BT_DIRECT;
+    const bool reordered = bt_mode == CUEV_BT_REORDERED;
+
+    CUEV_HIP(hipEventRecord(ev[0], s));
+    CUEV_TRY(dbbr_reduce<T>(ws, A, ws.B));
+    CUEV_HIP(hipEventRecord(ev[1], s));
+    CUEV_TRY(bc_chase<T>(ws, ws.B, ws.d, ws.e));
+    CUEV_HIP(hipEventRecord(ev[2], s));
+    // (d, e) to the host
+    CUEV_HIP(hipMemcpyAsync(ctx.h_d, ws.d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipMemcpyAsync(ctx.h_e, ws.e, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipEventRecord(ev[3], s));
+    if (reordered) {
+        // X = Q_s Q_b built on the GPU while the host runs the D&C
+        CUEV_TRY(fill2d<T>(s, ws.M, ws.ld, n, n, T(0), T(1)));
+        CUEV_TRY(bc_back<T>(ws, ws.U, ws.M, ws.ld));
+        CUEV_HIP(hipEventRecord(ev[4], s));
+        CUEV_TRY(sbr_back<T>(ws, ws.Y, ws.W, ws.M, ws.ld));
+        CUEV_HIP(hipEventRecord(ev[5], s));
+    }
+    // wait for (d, e) only; the reordered back-transform keeps running on the GPU.
+    // The device error flag is checked once the stream has drained.
+    CUEV_HIP(hipEventSynchronize(ev[3]));
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite((double)ctx.h_d[i]) || (i < n - 1 && !std::isfinite((double)ctx.h_e[i])))
+            CUEV_FAIL(CUEV_STATUS_NOT_FINITE,
+                      "tridiagonal matrix has a non-finite entry at %d (input not finite?)", i);
+    const auto dc0 = clk::now();
+    CUEV_TRY(stedc<T>(n, ctx.h_d, ctx.h_e, ctx.h_z, n, ctx.h_work, ctx.lwork, ctx.h_iwork,
+                      ctx.liwork));
+    const auto dc1 = clk::now();
+    CUEV_HIP(hipEventRecord(ev[6], s));
+    CUEV_HIP(hipMemcpyAsync(eval, ctx.h_d, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    Scratch gs{ws.gscratch, ws.gscratch_bytes};
+    if (reordered) {
+        // Q_d into W (free after sbr_back), then evec = X Q_d
+        CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
+        CUEV_HIP(hipEventRecord(ev[7], s));
+        CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), ws.M, ws.ld, ws.W, ws.ld, T(0), evec, n, gs));
+        CUEV_HIP(hipEventRecord(ev[8], s));
+    } else {
+        CUEV_HIP(hipMemcpyAsync(evec, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
+        CUEV_HIP(hipEventRecord(ev[7], s));
+        CUEV_TRY(bc_back<T>(ws, ws.U, evec, n));
+        CUEV_HIP(hipEventRecord(ev[4], s));
+        CUEV_TRY(sbr_back<T>(ws, ws.Y, ws.W, evec, n));
+        CUEV_HIP(hipEventRecord(ev[5], s));
+    }
+    CUEV_HIP(hipStreamSynchronize(s));
+    CUEV_TRY(workspace_check_error(ws));
+    const auto wall1 = clk::now();
+    if (timer) {
+        std::memset(timer, 0, sizeof(*timer));
+        timer->dbbr_ms = ev_ms(ev[0], ev[1]);
+        timer->bc_ms = ev_ms(ev[1], ev[2]);
+        timer->dc_ms = (float)ms_since(dc0, dc1);
+        timer->total_ms = (float)ms_since(wall0, wall1);
+        if (reordered) {
+            timer->bt_qb_ms = ev_ms(ev[3], ev[4]);
+            timer->bt_qs_ms = ev_ms(ev[4], ev[5]);
+            timer->overlap_ms = timer->bt_qb_ms + timer->bt_qs_ms;
+            timer->bt_copyin_ms = ev_ms(ev[6], ev[7]);
+            timer->bt_copyout_ms = ev_ms(ev[7], ev[8]);
+            timer->bt_ms = (float)ms_since(dc1, wall1);
+        } else {
+            timer->bt_copyin_ms = ev_ms(ev[6], ev[7]);
+            timer->bt_qb_ms = ev_ms(ev[7], ev[4]);
+            timer->bt_qs_ms = ev_ms(ev[4], ev[5]);
+            timer->bt_ms = (float)ms_since(dc1, wall1);
+        }
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
+template int tridi_dc<float>(Context<float> &, float *, float *, float *, float *, i64, double *);
+template int tridi_dc<double>(Context<double> &, double *, double *, double *, double *, i64,
+                              double *);
+template int back_transform<float>(Context<float> &, const float *, const float *, const float *,
+                                   float *, cuev_solve_timer *);
+template int back_transform<double>(Context<double> &, const double *, const double *,
+                                    const double *, double *, cuev_solve_timer *);
+template int solve_native<float>(Context<float> &, float *, float *, float *, cuev_solve_timer *,
+                                 int);
+template int solve_native<double>(Context<double> &, double *, double *, double *,
+                                  cuev_solve_timer *, int);
+
+} // namespace cuev

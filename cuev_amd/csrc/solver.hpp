@@ -1,0 +1,21 @@
+// solver.hpp — stage orchestration entry points.
+#pragma once
+
+#include "context.hpp"
+
+namespace cuev {
+
+void options_defaults(cuev_options &o);
+
+// Host D&C on device (d, e): eval <- eigenvalues, evec (ld ldv) <- Q_d.
+template <typename T>
+int tridi_dc(Context<T> &ctx, T *d, T *e, T *eval, T *evec, i64 ldv, double *dc_ms);
+// evec <- Q_s Q_b evec (direct order).
+template <typename T>
+int back_transform(Context<T> &ctx, const T *Y, const T *W, const T *U, T *evec,
+                   cuev_solve_timer *timer);
+// Full solve in the precision T.
+template <typename T>
+int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode);
+
+} // namespace cuev

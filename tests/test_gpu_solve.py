@@ -1,0 +1,145 @@
+"""End-to-end parity of cuev_{s,d}symm_eig_solve with host LAPACK dsyevd.
+
+Tolerances are the north-star ones: eigenvalues within 1e-12 ||A||_2 of dsyevd,
+||AV - V diag(lambda)||_F / (n ||A||_F) <= 1e-13 and ||V^T V - I||_F / n <= 1e-13.
+Inputs: the bench_solver fill (bench/bench_solver.cpp:53-58), the reference's
+glibc test fill, the n=1024 configuration A = (G + G^T)/2 with seed 0, edge
+sizes (n <= 2, n not a multiple of b), structured matrices (zero, diagonal,
+already tridiagonal, repeated eigenvalues) and every bandwidth / back-transform
+order.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import cuev_amd as cv
+from helpers import to_dev, eig_metrics
+
+pytestmark = pytest.mark.gpu
+
+
+def solve_np(torch, A, prec="d", **kw):
+    dt = torch.float64 if prec == "d" else torch.float32
+    dA = to_dev(A, dt)
+    lam, V = cv.symm_eig_solve(dA, overwrite=True, **kw)
+    torch.cuda.synchronize()
+    return lam.double().cpu().numpy(), V.double().cpu().numpy()
+
+
+def check(A, lam, V, ev_tol=1e-12, res_tol=1e-13, orth_tol=1e-13):
+    lam_ref = sl.eigh(A, eigvals_only=True, driver="evd")
+    ev_err, res, orth = eig_metrics(A, lam, V, lam_ref)
+    assert np.all(np.diff(lam) >= 0), "eigenvalues must be ascending"
+    assert ev_err <= ev_tol, f"eigenvalue error {ev_err:.2e}"
+    assert res <= res_tol, f"residual {res:.2e}"
+    assert orth <= orth_tol, f"orthogonality {orth:.2e}"
+    return ev_err, res, orth
+
+
+def test_config_n1024_seed0(torch_dev):
+    """n=1024, A = (G + G^T)/2, G iid N(0,1) from seed 0, b=32, k=256."""
+    rng = np.random.default_rng(0)
+    G = rng.standard_normal((1024, 1024))
+    A = (G + G.T) / 2
+    lam, V = solve_np(torch_dev, A, bandwidth=32, block=256)
+    check(A, lam, V)
+
+
+@pytest.mark.parametrize("n", [500, 1000])
+def test_bench_solver_fill(torch_dev, n):
+    from oracle import oracle as O
+    A = O.fill_glibc_symmetric(n, 1234)
+    lam, V = solve_np(torch_dev, A)
+    check(A, lam, V)
+
+
+@pytest.mark.parametrize("n,b,k,bt", [(1, 32, 512, "auto"), (2, 32, 512, "auto"), (3, 16, 16, "auto"),
+                                       (5, 32, 512, "direct"), (33, 32, 64, "direct"),
+                                       (65, 64, 128, "reordered"), (130, 16, 32, "direct"),
+                                       (257, 32, 96, "reordered"), (700, 64, 512, "direct"),
+                                       (700, 64, 512, "reordered"), (2500, 64, 512, "auto"),
+                                       (3001, 32, 512, "reordered")])
+def test_sizes_and_orders(torch_dev, n, b, k, bt):
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    A = (G + G.T) / 2
+    lam, V = solve_np(torch_dev, A, bandwidth=b, block=k, bt_mode=bt)
+    check(A, lam, V)
+
+
+def test_float32(torch_dev):
+    from oracle import oracle as O
+    A = O.sym_from_lower(O.fill_glibc_matrix(384, 384, 7))
+    lam, V = solve_np(torch_dev, A, prec="s", bandwidth=32, block=96)
+    check(A, lam, V, ev_tol=1e-5, res_tol=1e-7, orth_tol=1e-7)
+
+
+@pytest.mark.parametrize("kind", ["zero", "diagonal", "tridiagonal", "repeated", "rank1"])
+def test_structured(torch_dev, kind):
+    n = 300
+    rng = np.random.default_rng(5)
+    if kind == "zero":
+        A = np.zeros((n, n))
+    elif kind == "diagonal":
+        A = np.diag(rng.standard_normal(n))
+    elif kind == "tridiagonal":
+        A = np.diag(rng.standard_normal(n)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    elif kind == "repeated":
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        lam = np.repeat([-1.0, 0.5, 2.0], n // 3)
+        A = (Q * lam) @ Q.T
+        A = (A + A.T) / 2
+    else:
+        v = rng.standard_normal(n)
+        A = np.outer(v, v)
+    lam, V = solve_np(torch_dev, A, bandwidth=32, block=128)
+    check(A, lam, V)
+
+
+def test_lower_triangle_only(torch_dev):
+    """Only the lower triangle is read: garbage in the upper triangle must not matter."""
+    n = 400
+    rng = np.random.default_rng(9)
+    G = rng.standard_normal((n, n))
+    A = (G + G.T) / 2
+    Ag = np.tril(A) + np.triu(np.full((n, n), 7.7e5), 1)
+    lam, V = solve_np(torch_dev, Ag)
+    check(A, lam, V)
+
+
+def test_invalid_arguments(torch_dev):
+    torch = torch_dev
+    L = cv.lib()
+    buf = torch.zeros(16, dtype=torch.float64, device="cuda")
+    p = ctypes.c_void_p(buf.data_ptr())
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert L.cuev_dsymm_eig_solve(p, 0, p, p, s, None) == 1
+    assert L.cuev_dsymm_eig_solve(None, 4, p, p, s, None) == 1
+    o = cv.Options()
+    L.cuev_options_init(ctypes.byref(o))
+    o.bandwidth = 48
+    assert L.cuev_dsymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(o)) == 1
+    o.bandwidth, o.block = 32, 48
+    assert L.cuev_dsymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(o)) == 1
+    assert L.cuev_ssymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(cv.Options(0, 0, 1, 0, 0, 0.0))) == 8
+
+
+def test_non_finite_input(torch_dev):
+    A = np.eye(50)
+    A[10, 3] = np.nan
+    with pytest.raises(cv.CuevError) as ei:
+        solve_np(torch_dev, A)
+    assert ei.value.status == 7
+
+
+def test_timer_fields(torch_dev):
+    torch = torch_dev
+    rng = np.random.default_rng(1)
+    G = rng.standard_normal((2048, 2048))
+    A = (G + G.T) / 2
+    dA = to_dev(A)
+    lam, V, t = cv.symm_eig_solve(dA, timer=True, bt_mode="reordered")
+    assert t["total_ms"] > 0 and t["dbbr_ms"] > 0 and t["bc_ms"] > 0 and t["dc_ms"] > 0
+    assert t["total_ms"] >= t["dc_ms"]
