@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""EVD wall time of the gfx950 two-stage symmetric eigensolver on one MI355X.
+
+Default workload: n = 16384, fp64, A = Q diag(lam) Q^T (Q: Householder-QR factor of
+a seeded Gaussian matrix, lam sorted iid U(-1,1)), bandwidth b = 64, block k = 512,
+all eigenvalues and eigenvectors. One step = one full cuev_dsymm_eig_solve call
+(DBBR -> bulge chasing -> host dstedc overlapped with the GPU build of Q_s Q_b ->
+final GEMM), input already resident in HBM; the copy of A that the solver consumes
+is made before each step's timer starts.
+
+Prints ONE JSON line: value = mean EVD wall seconds per solve (lower is better),
+per-stage times, correctness against the exact eigenvalues, the roofline of the
+dominant GPU kernel measured with HIP events inside the library, and host LAPACK
+dsyevd on a bounded sample as the CPU baseline. With --mixed the FP32 + FP64
+refinement form is timed on the same matrix as well.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix peak, FLOP/s
+PEAK_FP32_MFMA = 157.3e12
+PEAK_HBM = 8.0e12          # bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_sample, n_full, seed=1):
+    """Host LAPACK dsyevd (SciPy's OpenBLAS) on an n_sample matrix of the same construction."""
+    import numpy as np
+    import scipy.linalg as sl
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 0) for d in threadpool_info()
+                       if d.get("internal_api") == "openblas"] or [os.cpu_count()])
+    except Exception:
+        threads = os.cpu_count()
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n_sample, n_sample)))
+    lam = np.sort(rng.uniform(-1.0, 1.0, n_sample))
+    A = (Q * lam) @ Q.T
+    A = (A + A.T) / 2
+    del Q
+    t0 = time.perf_counter()
+    w, V = sl.eigh(A, driver="evd", overwrite_a=True, check_finite=False)
+    t = time.perf_counter() - t0
+    scaled = t * (n_full / n_sample) ** 3
+    return {"value": round(scaled, 3), "unit": "s", "cores": int(threads), "kind": "port",
+            "sample": f"LAPACK dsyevd (jobz=V, SciPy OpenBLAS) on one n={n_sample} fp64 "
+                      f"Q diag(lam) Q^T matrix: {t:.2f} s measured, scaled by (n/{n_sample})^3 "
+                      f"to n={n_full}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="q16k", choices=["q16k", "n1024", "g49k", "mixed16k"])
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--b", type=int, default=64)
+    ap.add_argument("--k", type=int, default=512)
+    ap.add_argument("--bt", default="auto", choices=["auto", "direct", "reordered"])
+    ap.add_argument("--mixed", action="store_true", help="also time the mixed FP32+refinement form")
+    ap.add_argument("--cpu-n", type=int, default=6144, help="n of the dsyevd CPU sample (0: skip)")
+    ap.add_argument("--no-check", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the single-GPU "
+              "solver has no multi-GPU path; see DESIGN.md for how it would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cuev_amd as cv
+    from cuev_amd import workloads
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    n_default = {"q16k": 16384, "n1024": 1024, "g49k": 49152, "mixed16k": 16384}[a.workload]
+    n = a.n or n_default
+    b, k = a.b, a.k
+    if a.workload == "n1024" and not a.n:
+        b, k = 32, 256
+    log(f"[bench] generating workload {a.workload} n={n} on the GPU")
+    t0 = time.perf_counter()
+    A0, lam_exact = workloads.make(a.workload, n=n, device="cuda")
+    torch.cuda.synchronize()
+    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s; lapack={cv.lapack_name()}")
+
+    work = torch.empty_like(A0)
+
+    def one(precision="native", timer=True):
+        work.copy_(A0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        lam, V, tm = cv.symm_eig_solve(work, overwrite=True, bandwidth=b, block=k, bt_mode=a.bt,
+                                       precision=precision, timer=True)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, lam, V, tm
+
+    for i in range(a.warmup):
+        dt, _, _, _ = one()
+        log(f"[bench] warmup {i}: {dt:.3f} s")
+
+    cv.profile_enable(True)
+    cv.profile_read(reset=True)
+    times, timers = [], []
+    for i in range(a.steps):
+        dt, lam, V, tm = one()
+        times.append(dt)
+        timers.append(tm)
+        log(f"[bench] step {i}: {dt:.3f} s  dbbr={tm['dbbr_ms']:.1f} bc={tm['bc_ms']:.1f} "
+            f"dc={tm['dc_ms']:.1f} bt_after_dc={tm['bt_ms']:.1f} (overlapped {tm['overlap_ms']:.1f}) ms")
+    prof = cv.profile_read(reset=True)
+    cv.profile_enable(False)
+    K = len(times)
+    mean_s = sum(times) / K
+
+    # ---- correctness of the last solve
+    check = {}
+    if not a.no_check:
+        lam64 = lam.double()
+        Vd = V.double()
+        An = torch.linalg.matrix_norm(A0).item()
+        R = A0 @ Vd - Vd * lam64
+        res = torch.linalg.matrix_norm(R).item() / (n * An)
+        del R
+        G = Vd.T @ Vd
+        G.diagonal().sub_(1.0)
+        orth = torch.linalg.matrix_norm(G).item() / n
+        del G
+        a2 = lam64.abs().max().item()
+        check = {"residual": res, "orthogonality": orth,
+                 "residual_bench_solver_form": res * n}
+        if lam_exact is not None:
+            check["eig_err_vs_exact_over_norm2"] = (lam64 - lam_exact).abs().max().item() / a2
+        check["pass_1e-13"] = bool(res <= 1e-13 and orth <= 1e-13)
+        log(f"[bench] residual {res:.2e} orth {orth:.2e} {check}")
+
+    # ---- per-kernel rates (library HIP events over the timed steps)
+    kernels = {}
+    for tag, r in prof.items():
+        per_launch_ms = r["ms"] / max(1, r["count"])
+        ent = {"ms_per_step": r["ms"] / K, "launches_per_step": r["count"] / K,
+               "avg_launch_ms": per_launch_ms}
+        if r["flops"] > 0 and r["ms"] > 0:
+            ent["tflops"] = r["flops"] / (r["ms"] * 1e-3) / 1e12
+        if r["bytes"] > 0 and r["ms"] > 0:
+            ent["gbps"] = r["bytes"] / (r["ms"] * 1e-3) / 1e9
+        kernels[tag] = ent
+
+    # dominant single-kernel launch site by GPU time per step
+    single = {"bc.chase": "hbm", "bt.bc_back": "mfma", "bt.gemm": "mfma", "dbbr.syr2k": "mfma",
+              "dbbr.symm": "mfma"}
+    dom = max((t for t in single if t in prof), key=lambda t: prof[t]["ms"], default=None)
+    roofline = None
+    if dom:
+        r = prof[dom]
+        dur = r["ms"] / r["count"] * 1e-3
+        if single[dom] == "hbm":
+            per_launch = r["bytes"] / r["count"]
+            ach = per_launch / dur / 1e9
+            roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(ach / 8000.0, 4), "traffic": None,
+                        "avg_launch_ms": round(dur * 1e3, 3)}
+        else:
+            per_launch = r["flops"] / r["count"]
+            ach = per_launch / dur / 1e12
+            roofline = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
+                        "avg_launch_ms": round(dur * 1e3, 3)}
+
+    # ---- stages (means over the timed steps)
+    stages = {key: sum(t[key] for t in timers) / K for key in
+              ("dbbr_ms", "bc_ms", "dc_ms", "bt_ms", "total_ms", "bt_copyin_ms", "bt_qb_ms",
+               "bt_qs_ms", "bt_copyout_ms", "overlap_ms")}
+    flops16_3 = 16.0 / 3.0 * n ** 3
+    dbbr_tf = (4.0 / 3.0 * n ** 3) / (stages["dbbr_ms"] * 1e-3) / 1e12 if stages["dbbr_ms"] else None
+
+    mixed = None
+    if a.mixed:
+        try:
+            for i in range(a.warmup):
+                one("mixed")
+            mt = []
+            for i in range(a.steps):
+                dt, lam_m, V_m, tm_m = one("mixed")
+                mt.append(dt)
+            Vd = V_m.double()
+            R = A0 @ Vd - Vd * lam_m.double()
+            res_m = torch.linalg.matrix_norm(R).item() / (n * torch.linalg.matrix_norm(A0).item())
+            mixed = {"seconds": sum(mt) / len(mt), "speedup_vs_fp64": mean_s / (sum(mt) / len(mt)),
+                     "residual": res_m, "refine_sweeps": tm_m["refine_sweeps"]}
+        except cv.CuevError as e:
+            mixed = {"error": str(e)}
+
+    cpu = None
+    if a.cpu_n:
+        log(f"[bench] CPU baseline: dsyevd n={a.cpu_n}")
+        del work
+        cpu = cpu_baseline(min(a.cpu_n, n), n)
+        log(f"[bench] CPU baseline {cpu}")
+
+    out = {
+        "metric": "EVD wall s at n=16k fp64 (two-stage, all eigenpairs)",
+        "value": round(mean_s, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": round(mean_s * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{a.workload}: n={n}, b={b}, k={k}, A=Q diag(lam) Q^T" if a.workload == "q16k"
+                   else f"{a.workload}: n={n}, b={b}, k={k}", "n": n, "b": b, "k": k,
+                   "bt_mode": a.bt, "parallelism": "single GPU"},
+        "stages_ms": {kk: round(v, 2) for kk, v in stages.items()},
+        "effective_tflops_16_3": round(flops16_3 / mean_s / 1e12, 3),
+        "effective_tflops_4_3": round(flops16_3 / 4.0 / mean_s / 1e12, 3),
+        "dbbr_tflops": round(dbbr_tf, 3) if dbbr_tf else None,
+        "dbbr_frac_fp64_peak": round(dbbr_tf / 78.6, 4) if dbbr_tf else None,
+        "check": check,
+        "kernels": {t: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in e.items()}
+                    for t, e in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+        "lapack": cv.lapack_name(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,7 +17,7 @@ import os
 __all__ = [
     "CuevError", "SolveTimer", "Options", "lib", "lib_path", "symm_eig_solve",
     "eig_backward", "Handle", "gemm", "symm_lower", "syr2k_lower", "lapack_name",
-    "STATUS", "BUF",
+    "STATUS", "BUF", "profile_enable", "profile_read",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -98,6 +98,8 @@ def _declare(L):
         getattr(L, f"cuev_{pre}gemm").argtypes = [P, I, I, I64, I64, I64, sc, P, I64, P, I64, sc, P, I64]
         getattr(L, f"cuev_{pre}symm_lower").argtypes = [P, I64, I64, sc, P, I64, P, I64, sc, P, I64]
         getattr(L, f"cuev_{pre}syr2k_lower").argtypes = [P, I64, I64, sc, P, I64, P, I64, sc, P, I64]
+    L.cuev_profile_enable.argtypes = [I]
+    L.cuev_profile_read.argtypes = [ctypes.c_char_p, I, I]
     L.cuev_handle_create.argtypes = [ctypes.POINTER(P), I, I, I, I, P]
     L.cuev_handle_destroy.argtypes = [P]
     L.cuev_handle_buffer.argtypes = [P, I, ctypes.POINTER(P), ctypes.POINTER(I64)]
@@ -107,6 +109,20 @@ def _check(st):
     if st != 0:
         msg = lib().cuev_last_error_message().decode(errors="replace")
         raise CuevError(st, msg)
+
+
+def profile_enable(on=True):
+    """Record per-launch HIP event timings inside the library (see profile_read)."""
+    lib().cuev_profile_enable(1 if on else 0)
+
+
+def profile_read(reset=True):
+    """Per-tag totals {tag: {ms, count, flops, bytes, max_ms}} of the recorded launches."""
+    import json
+    need = lib().cuev_profile_read(None, 0, 0)
+    buf = ctypes.create_string_buffer(max(need, 2) + 64)
+    lib().cuev_profile_read(buf, len(buf), 1 if reset else 0)
+    return json.loads(buf.value.decode() or "{}")
 
 
 def lapack_name():

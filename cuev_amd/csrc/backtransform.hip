@@ -29,6 +29,7 @@
 #include <algorithm>
 
 #include "gemm.hpp"
+#include "profile.hpp"
 
 namespace cuev {
 namespace {
@@ -148,6 +149,8 @@ template <typename T, int B> int launch_bc_back(Workspace<T> &ws, const T *U, T 
     const int waves_per_block = kBTThreads / 64;
     const i64 max_blocks = (i64)std::max(1, per_cu) * device_cu_count();
     const i64 blocks = std::min<i64>(max_blocks, ceil_div(n, waves_per_block));
+    // 4 flops per reflector entry per column: sum_s (n-1-s) entries ~ n^2/2
+    prof::Scope sc(ws.stream, "bt.bc_back", 2.0 * (double)n * n * n);
     hipLaunchKernelGGL((bc_back_kernel<T, B>), dim3((unsigned)blocks), dim3(kBTThreads), 0,
                        ws.stream, U, ws.ldu, M, ldm, n, n);
     CUEV_LAUNCH_CHECK();
@@ -180,6 +183,7 @@ template <typename T> int sbr_back(Workspace<T> &ws, const T *Y, const T *W, T *
         const T *Wp = W + (i64)j * ld + (j + b);
         T *Mb = M + (j + b);
         // Zt = Y_j^T M_b (b x n) ; M_b -= W_j Zt
+        prof::Scope sc(ws.stream, "bt.sbr_back", 4.0 * rows * (double)b * n);
         CUEV_TRY(gemm<T>(ws.stream, 1, 0, b, n, rows, T(1), Yp, ld, Mb, ldm, T(0), ws.Z, b, gs));
         CUEV_TRY(gemm<T>(ws.stream, 0, 0, rows, n, b, T(-1), Wp, ld, ws.Z, b, T(1), Mb, ldm, gs));
     }

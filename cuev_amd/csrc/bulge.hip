@@ -32,6 +32,7 @@
 
 #include <algorithm>
 
+#include "profile.hpp"
 #include "util.hpp"
 
 namespace cuev {
@@ -214,6 +215,10 @@ template <typename T> int bc_chase(Workspace<T> &ws, T *Bp, T *d, T *e) {
         // at most ~n/(2b) sweeps can be active at once; more workgroups would only spin
         const int useful = (int)ceil_div(n, 2 * b) + 2;
         const int grid = std::max(1, std::min({nsweeps, resident, useful}));
+        // algorithmic work: ~n^2/(2b) hops, each reading and writing a 2b x 2b window
+        // of the packed band (8 b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops
+        const double hops = (double)n * n / (2.0 * b);
+        prof::Scope sc(s, "bc.chase", hops * (6.0 * b * b + 16.0 * b * b), hops * 8.0 * b * b * sizeof(T));
         hipLaunchKernelGGL(bc_chase_kernel<T>, dim3(grid), dim3(kBCThreads), lds, s, Bp, ws.U,
                            ws.ldu, n, b, ws.prog, ws.err);
         CUEV_LAUNCH_CHECK();

@@ -33,7 +33,7 @@ template <typename T> int context_create(Context<T> *&out, int n, int b, int k, 
         ctx->lwork = stedc_lwork(n);
         ctx->liwork = stedc_liwork(n);
         ctx->h_work = static_cast<T *>(std::malloc(sizeof(T) * (size_t)ctx->lwork));
-        ctx->h_iwork = static_cast<int *>(std::malloc(sizeof(int) * (size_t)ctx->liwork));
+        ctx->h_iwork = static_cast<int64_t *>(std::malloc(sizeof(int64_t) * (size_t)ctx->liwork));
         if (!ctx->h_work || !ctx->h_iwork) {
             detail::set_error("host malloc of the dstedc workspace failed (n=%d)", n);
             st = CUEV_STATUS_ALLOC_FAILED;

@@ -13,9 +13,9 @@ template <typename T> struct Context {
     T *h_z = nullptr;     // n x n (Q_d)
     // host LAPACK workspace (pageable)
     T *h_work = nullptr;
-    int *h_iwork = nullptr;
+    int64_t *h_iwork = nullptr;
     i64 lwork = 0;
-    int liwork = 0;
+    i64 liwork = 0;
     hipEvent_t ev[12] = {};
     int dtype = 0;
 };

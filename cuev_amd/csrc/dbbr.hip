@@ -18,6 +18,7 @@
 
 #include "gemm.hpp"
 #include "panel_qr.hpp"
+#include "profile.hpp"
 
 namespace cuev {
 namespace {
@@ -40,6 +41,7 @@ __global__ __launch_bounds__(256) void bc_pack_kernel(const T *A, i64 lda, T *Bp
 template <typename T> int dbbr_pack(Workspace<T> &ws, const T *A, T *Bp, int n, int b) {
     if (n < 1 || b < 1) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "dbbr_pack: n=%d b=%d", n, b);
     const i64 total = (i64)2 * b * n;
+    prof::Scope sc(ws.stream, "dbbr.pack", 0.0, (double)sizeof(T) * ((double)(b + 1) * n + 2.0 * b * n));
     hipLaunchKernelGGL(bc_pack_kernel<T>, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0,
                        ws.stream, A, ws.ld, Bp, n, b);
     CUEV_LAUNCH_CHECK();
@@ -71,13 +73,20 @@ template <typename T> int dbbr_reduce(Workspace<T> &ws, T *A, T *Bp) {
             const T *Yb = Yblk_base + (j + b);            // Y_blk restricted to rows j+b..
             const T *Zb = ws.Z + (j + b);                 // Z_blk restricted to rows j+b..
 
+            const double R = rows, Bw = b;
             // panel QR, explicit Y, T and W = Y T
-            CUEV_TRY(dbbr_panel_qr(ws, A + off, ws.Y + off, ws.W + off, rows, b));
-
+            {
+                prof::Scope sc(s, "dbbr.panel_qr", 2.0 * R * Bw * Bw + 2.0 * R * Bw * Bw / 2.0);
+                CUEV_TRY(dbbr_panel_qr(ws, A + off, ws.Y + off, ws.W + off, rows, b));
+            }
             // P = A22 W
-            CUEV_TRY(symm_lower<T>(s, rows, b, one, A + tr, lda, ws.W + off, lda, zero, P, lda, gs));
+            {
+                prof::Scope sc(s, "dbbr.symm", 2.0 * R * R * Bw);
+                CUEV_TRY(symm_lower<T>(s, rows, b, one, A + tr, lda, ws.W + off, lda, zero, P, lda, gs));
+            }
 
             if (w > 0) {
+                prof::Scope sc(s, "dbbr.corr", 8.0 * R * Bw * w);
                 // D = [Z_blk^T W ; Y_blk^T W]   (2w x b, ld 2w)
                 const i64 ldd = 2 * w;
                 CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Zb, lda, ws.W + off, lda, zero, ws.Dwk,
@@ -91,14 +100,18 @@ template <typename T> int dbbr_reduce(Workspace<T> &ws, T *A, T *Bp) {
             }
 
             // C = W^T P  (b x b) ; Z = P - 1/2 Y C
-            CUEV_TRY(gemm<T>(s, 1, 0, b, b, rows, one, ws.W + off, lda, P, lda, zero, ws.Dwk, b, gs));
-            CUEV_TRY(gemm<T>(s, 0, 0, rows, b, b, neg_half, ws.Y + off, lda, ws.Dwk, b, one, P, lda,
-                             gs));
+            {
+                prof::Scope sc(s, "dbbr.wtp_z", 4.0 * R * Bw * Bw);
+                CUEV_TRY(gemm<T>(s, 1, 0, b, b, rows, one, ws.W + off, lda, P, lda, zero, ws.Dwk, b, gs));
+                CUEV_TRY(gemm<T>(s, 0, 0, rows, b, b, neg_half, ws.Y + off, lda, ws.Dwk, b, one, P, lda,
+                                 gs));
+            }
             w += b;
 
             // bring the next panel's columns up to date with the whole block so far:
             // A[j+b:n, j+b:j+2b] -= Z_blk Y_blk[j+b:j+2b,:]^T + Y_blk Z_blk[j+b:j+2b,:]^T
             if (j + b < i + kc) {
+                prof::Scope sc(s, "dbbr.green", 2.0 * R * Bw * 2.0 * w);
                 CUEV_TRY(gemm_ex<T>(s, rows, b, 2 * w, neg1, opnd_cat(Zb, lda, Yb, lda, w, 0),
                                     opnd_cat(Yb, lda, Zb, lda, w, 1), one, A + tr, lda, 0, gs));
             }
@@ -107,6 +120,8 @@ template <typename T> int dbbr_reduce(Workspace<T> &ws, T *A, T *Bp) {
         // deferred trailing update: A[t:n, t:n] -= Y Z^T + Z Y^T (lower)
         const int t = i + kc;
         if (t < n && w > 0) {
+            const double nt = n - t;
+            prof::Scope sc(s, "dbbr.syr2k", 2.0 * nt * (nt + 1) * w);
             CUEV_TRY(syr2k_lower<T>(s, n - t, w, neg1, Yblk_base + t, lda, ws.Z + t, lda, one,
                                     A + (i64)t * lda + t, lda, gs));
         }

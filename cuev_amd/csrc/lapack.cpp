@@ -4,11 +4,15 @@
 // calls ?steqr/?laed2/?laed4/?lamrg (src/cuda/tridi.cu:38-53). Here the
 // tridiagonal eigensolver is LAPACK ?stedc itself, located at run time:
 //   1. $CUEV_LAPACK_LIB (a path or soname), if set;
-//   2. Intel MKL's single dynamic library (libmkl_rt);
-//   3. the OpenBLAS build shipped with SciPy (symbols prefixed "scipy_");
+//   2. the OpenBLAS build shipped with SciPy (every symbol prefixed "scipy_");
+//   3. Intel MKL's single dynamic library (libmkl_rt);
 //   4. a system liblapack / libopenblas.
-// The first library that exports dstedc (and sstedc) wins; its name is kept for
-// reports (cuev_lapack_name()).
+// The prefixed OpenBLAS comes before MKL on purpose: in a process that has
+// loaded PyTorch, torch's bundled libraries export unprefixed dgemm_/dlaed4_,
+// which interpose on MKL's internal calls (wrong eigenvectors or a crash); the
+// prefixed build cannot be interposed. For n > 46340 the ILP64 build that
+// NumPy ships (symbols "scipy_*_64_") supplies dstedc with 64-bit integers.
+// The library's name is kept for reports (cuev_lapack_name()).
 #include "lapack.hpp"
 
 #include <dlfcn.h>
@@ -57,6 +61,26 @@ std::vector<std::string> glob_paths(const char *pattern) {
     return out;
 }
 
+void try_ilp64() {
+    for (const char *pat : {"/usr/local/lib/python3*/dist-packages/numpy.libs/libscipy_openblas64_*.so",
+                            "/usr/lib/python3*/dist-packages/numpy.libs/libscipy_openblas64_*.so",
+                            "/usr/local/lib/python3*/site-packages/numpy.libs/libscipy_openblas64_*.so"}) {
+        for (const auto &p : glob_paths(pat)) {
+            void *h = dlopen(p.c_str(), RTLD_NOW | RTLD_LOCAL);
+            if (!h) continue;
+            void *d = dlsym(h, "scipy_dstedc_64_");
+            void *s = dlsym(h, "scipy_sstedc_64_");
+            if (d && s) {
+                g_fns.dstedc64 = reinterpret_cast<LapackFns::dstedc64_t>(d);
+                g_fns.sstedc64 = reinterpret_cast<LapackFns::sstedc64_t>(s);
+                g_fns.name64 = p;
+                return;
+            }
+            dlclose(h);
+        }
+    }
+}
+
 } // namespace
 
 const LapackFns *lapack() {
@@ -64,50 +88,70 @@ const LapackFns *lapack() {
     if (g_tried) return g_fns.dstedc ? &g_fns : nullptr;
     g_tried = true;
     if (const char *env = std::getenv("CUEV_LAPACK_LIB")) {
-        if (try_lib(env, "") || try_lib(env, "scipy_")) return &g_fns;
+        if (try_lib(env, "") || try_lib(env, "scipy_")) {
+            try_ilp64();
+            return &g_fns;
+        }
     }
-    for (const char *p : {"libmkl_rt.so.2", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.2",
-                          "/opt/conda/lib/libmkl_rt.so"})
-        if (try_lib(p, "")) return &g_fns;
+    bool ok = false;
     for (const char *pat : {"/usr/local/lib/python3*/dist-packages/scipy.libs/libscipy_openblas*.so",
                             "/usr/lib/python3*/dist-packages/scipy.libs/libscipy_openblas*.so",
                             "/usr/local/lib/python3*/site-packages/scipy.libs/libscipy_openblas*.so"})
         for (const auto &p : glob_paths(pat))
-            if (try_lib(p, "scipy_")) return &g_fns;
-    for (const char *p : {"liblapack.so.3", "liblapack.so", "libopenblas.so.0", "libopenblas.so"})
-        if (try_lib(p, "")) return &g_fns;
-    return nullptr;
+            if (!ok && try_lib(p, "scipy_")) ok = true;
+    if (!ok)
+        for (const char *p : {"libmkl_rt.so.2", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.2",
+                              "/opt/conda/lib/libmkl_rt.so"})
+            if (!ok && try_lib(p, "")) ok = true;
+    if (!ok)
+        for (const char *p : {"liblapack.so.3", "liblapack.so", "libopenblas.so.0", "libopenblas.so"})
+            if (!ok && try_lib(p, "")) ok = true;
+    try_ilp64();
+    return ok ? &g_fns : nullptr;
+}
+
+template <typename T, typename F32, typename F64>
+static int stedc_impl(F32 f32, F64 f64, const char *what, int n, T *d, T *e, T *z, int ldz, T *work,
+                      i64 lwork, int64_t *iwork, i64 liwork) {
+    const char compz = 'I';
+    if (lwork <= 0x7fffffffLL && liwork <= 0x7fffffffLL && f32) {
+        // LP64: 32-bit integer workspace lives in the first half of iwork
+        int *iw = reinterpret_cast<int *>(iwork);
+        const int lw = (int)lwork, liw = (int)liwork;
+        int info = 0;
+        f32(&compz, &n, d, e, z, &ldz, work, &lw, iw, &liw, &info);
+        if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "%s returned info = %d (n = %d)", what, info, n);
+        return CUEV_STATUS_SUCCESS;
+    }
+    if (!f64)
+        CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE,
+                  "%s at n = %d needs a 64-bit-integer LAPACK; none was found", what, n);
+    const int64_t n64 = n, ldz64 = ldz, lw64 = lwork, liw64 = liwork;
+    int64_t info = 0;
+    f64(&compz, &n64, d, e, z, &ldz64, work, &lw64, iwork, &liw64, &info);
+    if (info != 0)
+        CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "%s (ILP64) returned info = %lld (n = %d)", what,
+                  (long long)info, n);
+    return CUEV_STATUS_SUCCESS;
 }
 
 template <> int stedc<double>(int n, double *d, double *e, double *z, int ldz, double *work,
-                              i64 lwork, int *iwork, int liwork) {
+                              i64 lwork, int64_t *iwork, i64 liwork) {
     const LapackFns *L = lapack();
     if (!L) CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE, "no host LAPACK exporting dstedc was found");
-    if (lwork > 0x7fffffffLL)
-        CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "dstedc workspace exceeds the LP64 LAPACK limit (n=%d)", n);
-    const int lw = (int)lwork;
-    int info = 0;
-    const char compz = 'I';
-    L->dstedc(&compz, &n, d, e, z, &ldz, work, &lw, iwork, &liwork, &info);
-    if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "dstedc returned info = %d (n = %d)", info, n);
-    return CUEV_STATUS_SUCCESS;
+    return stedc_impl<double>(L->dstedc, L->dstedc64, "dstedc", n, d, e, z, ldz, work, lwork, iwork,
+                              liwork);
 }
 
 template <> int stedc<float>(int n, float *d, float *e, float *z, int ldz, float *work, i64 lwork,
-                             int *iwork, int liwork) {
+                             int64_t *iwork, i64 liwork) {
     const LapackFns *L = lapack();
     if (!L) CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE, "no host LAPACK exporting sstedc was found");
-    if (lwork > 0x7fffffffLL)
-        CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "sstedc workspace exceeds the LP64 LAPACK limit (n=%d)", n);
-    const int lw = (int)lwork;
-    int info = 0;
-    const char compz = 'I';
-    L->sstedc(&compz, &n, d, e, z, &ldz, work, &lw, iwork, &liwork, &info);
-    if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "sstedc returned info = %d (n = %d)", info, n);
-    return CUEV_STATUS_SUCCESS;
+    return stedc_impl<float>(L->sstedc, L->sstedc64, "sstedc", n, d, e, z, ldz, work, lwork, iwork,
+                             liwork);
 }
 
 i64 stedc_lwork(int n) { return n <= 1 ? 1 : 1 + 4 * (i64)n + (i64)n * n; }
-int stedc_liwork(int n) { return n <= 1 ? 1 : 3 + 5 * n; }
+i64 stedc_liwork(int n) { return n <= 1 ? 1 : 3 + 5 * (i64)n; }
 
 } // namespace cuev

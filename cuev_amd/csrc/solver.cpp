@@ -21,6 +21,7 @@
 #include "dbbr.hpp"
 #include "gemm.hpp"
 #include "lapack.hpp"
+#include "profile.hpp"
 #include "util.hpp"
 
 namespace cuev {
@@ -130,7 +131,10 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
         // Q_d into W (free after sbr_back), then evec = X Q_d
         CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipEventRecord(ev[7], s));
-        CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), ws.M, ws.ld, ws.W, ws.ld, T(0), evec, n, gs));
+        {
+            prof::Scope sc(s, "bt.gemm", 2.0 * (double)n * n * n);
+            CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), ws.M, ws.ld, ws.W, ws.ld, T(0), evec, n, gs));
+        }
         CUEV_HIP(hipEventRecord(ev[8], s));
     } else {
         CUEV_HIP(hipMemcpyAsync(evec, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));

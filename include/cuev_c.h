@@ -226,6 +226,18 @@ int cuev_ssyr2k_lower(hipStream_t stream, int64_t n, int64_t k, float alpha, con
                       int64_t lda, const float *B, int64_t ldb, float beta, float *C,
                       int64_t ldc);
 
+/* ------------------------------------------------------------------------ */
+/* Opt-in per-launch timing with HIP events (no reference counterpart; the   */
+/* reference times whole stages only, through SolveTimer). Also enabled by    */
+/* CUEV_PROFILE=1 in the environment.                                        */
+/* ------------------------------------------------------------------------ */
+int cuev_profile_enable(int on);
+/* Waits for the recorded events and writes per-tag totals as JSON,
+   {"tag": {"ms", "count", "flops", "bytes", "max_ms"}, ...}, into buf (len bytes).
+   Returns the length the JSON needs (including the terminating NUL).
+   reset != 0 clears the records. */
+int cuev_profile_read(char *buf, int len, int reset);
+
 #ifdef __cplusplus
 }
 #endif
