@@ -109,10 +109,14 @@ template <typename T> int dbbr_reduce(Workspace<T> &ws, T *A, T *Bp) {
             w += b;
 
             // bring the next panel's columns up to date with the whole block so far:
-            // A[j+b:n, j+b:j+2b] -= Z_blk Y_blk[j+b:j+2b,:]^T + Y_blk Z_blk[j+b:j+2b,:]^T
+            // A[j+b:n, j+b:j+b+nc] -= Z_blk Y_blk[j+b:j+b+nc,:]^T + Y_blk Z_blk[j+b:j+b+nc,:]^T
+            // The next panel has nc = min(b, n-j-b) columns: the reference updates b
+            // columns unconditionally (dbbr.cu:210-216), which writes past the end of
+            // A whenever n - j - b < b in the last block.
             if (j + b < i + kc) {
-                prof::Scope sc(s, "dbbr.green", 2.0 * R * Bw * 2.0 * w);
-                CUEV_TRY(gemm_ex<T>(s, rows, b, 2 * w, neg1, opnd_cat(Zb, lda, Yb, lda, w, 0),
+                const int nc = std::min(b, n - j - b);
+                prof::Scope sc(s, "dbbr.green", 2.0 * R * nc * 2.0 * w);
+                CUEV_TRY(gemm_ex<T>(s, rows, nc, 2 * w, neg1, opnd_cat(Zb, lda, Yb, lda, w, 0),
                                     opnd_cat(Yb, lda, Zb, lda, w, 1), one, A + tr, lda, 0, gs));
             }
         }

@@ -168,8 +168,11 @@ void orc_dbbr_reduce(int n, int b, int k, double *A, double *Y, double *W, doubl
             gemm(0, 0, rows, b, b, -0.5, Y + off, lda, C, b, 1.0, P, lda);      /* Z = P - Y C/2 */
             w += b;
             if (j + b < i + kc) { /* green update (dbbr.cu:208-217) */
-                gemm(0, 1, rows, b, w, -1.0, Zb, lda, Yb, lda, 1.0, A + tr, lda);
-                gemm(0, 1, rows, b, w, -1.0, Yb, lda, Zb, lda, 1.0, A + tr, lda);
+                /* only the nc = min(b, n-j-b) columns that exist: the reference updates b
+                   columns, past the end of A when the last block is ragged */
+                const int nc = (b < n - j - b) ? b : n - j - b;
+                gemm(0, 1, rows, nc, w, -1.0, Zb, lda, Yb, lda, 1.0, A + tr, lda);
+                gemm(0, 1, rows, nc, w, -1.0, Yb, lda, Zb, lda, 1.0, A + tr, lda);
             }
         }
         const int t = i + kc;

@@ -185,3 +185,23 @@ def test_tridi_dc(torch_dev, n, prec, res_tol, orth_tol):
     T = O.tridiag_dense(d, e)
     assert np.linalg.norm(T @ Vh - Vh * lam) / np.linalg.norm(T) < res_tol
     assert np.linalg.norm(Vh.T @ Vh - np.eye(n)) / np.sqrt(n) < orth_tol
+
+
+@pytest.mark.parametrize("n,b,k", [(250, 64, 128), (2500, 64, 512), (300, 16, 64)])
+def test_dbbr_ragged_bounds(torch_dev, n, b, k):
+    """Regression: the next-panel update of a ragged last block stays inside A.
+
+    A occupies the front of a larger device buffer whose tail holds a sentinel.
+    """
+    torch = torch_dev
+    A0 = O.fill_glibc_matrix(n, n, 7)
+    flat = torch.full((n * n + 2 * b * n,), 12345.0, dtype=torch.float64, device="cuda")
+    flat[: n * n] = torch.from_numpy(A0.ravel(order="F")).cuda()
+    with cv.Handle(torch.float64, n, b, k) as h:
+        h.dbbr_reduce(flat)
+        torch.cuda.synchronize()
+    tail = flat[n * n:].cpu().numpy()
+    assert np.all(tail == 12345.0)
+    Ar = flat[: n * n].cpu().numpy().reshape((n, n), order="F")
+    ev = np.linalg.eigvalsh(O.sym_from_lower(A0))
+    assert np.abs(ev - np.linalg.eigvalsh(O.band_dense(Ar, b))).max() / (ev[-1] - ev[0]) < 1e-9

@@ -152,3 +152,28 @@ def test_full_pipeline_oracle_vs_dsyevd():
     lam_ref = sl.eigh(As, eigvals_only=True, driver="evd")
     assert np.abs(lam - lam_ref).max() / np.abs(lam_ref).max() < 1e-13
     assert np.linalg.norm(As @ V - V * lam) / (n * np.linalg.norm(As)) < 1e-15
+
+
+@pytest.mark.parametrize("n,b,k", [(250, 64, 128), (300, 16, 64), (97, 16, 32), (200, 64, 128)])
+def test_dbbr_ragged_last_block_stays_in_bounds(n, b, k):
+    """Regression: the next-panel update must not touch columns >= n when n - j - b < b.
+
+    A is stored at the front of a larger buffer whose tail (where column n would
+    start, ld = n) holds a sentinel; a write past column n-1 changes it. Y, W, Z
+    start as garbage: the reduction may only read entries it has written, and an
+    out-of-range update then writes a value that differs from the sentinel.
+    """
+    import ctypes
+    A0 = O.fill_glibc_matrix(n, n, 7)
+    flat = np.full(n * n + 2 * b * n, 12345.0)
+    flat[: n * n] = A0.ravel(order="F")
+    Y = np.full((n, n), 7.0, order="F")
+    W = np.full((n, n), 7.0, order="F")
+    Z = np.full((n, k), 7.0, order="F")
+    Bp = np.zeros((2 * b, n), order="F")
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    O.lib().orc_dbbr_reduce(n, b, k, vp(flat), vp(Y), vp(W), vp(Z), vp(Bp))
+    assert np.all(flat[n * n:] == 12345.0), "wrote past the end of A"
+    Ar = flat[: n * n].reshape((n, n), order="F")
+    ev = np.linalg.eigvalsh(O.sym_from_lower(A0))
+    assert np.abs(ev - np.linalg.eigvalsh(O.band_dense(Ar, b))).max() / (ev[-1] - ev[0]) < 1e-13
