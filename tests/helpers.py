@@ -48,8 +48,10 @@ def eig_metrics(A, lam, V, lam_ref):
     """BASELINE metrics: eigenvalue error / ||A||_2, residual ||AV - V diag(lam)||_F/(n ||A||_F),
     orthogonality ||V^T V - I||_F / n."""
     n = A.shape[0]
-    a2 = np.max(np.abs(lam_ref)) if len(lam_ref) else 1.0
+    a2 = np.max(np.abs(lam_ref)) if len(lam_ref) else 0.0
+    a2 = a2 if a2 > 0 else 1.0  # zero matrix: absolute errors
     afro = np.linalg.norm(A)
+    afro = afro if afro > 0 else 1.0
     ev_err = np.max(np.abs(lam - lam_ref)) / a2
     res = np.linalg.norm(A @ V - V * lam) / (n * afro)
     orth = np.linalg.norm(V.T @ V - np.eye(n)) / n
