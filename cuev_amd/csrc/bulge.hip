@@ -13,10 +13,12 @@
 //  * Reflector convention: if the vector below the pivot is exactly zero the
 //    hop is the identity (w = 0), as LAPACK's dlarfg does; the reference's
 //    normalised reflector would flip the sign of the pivot instead.
-//  * Each hop works on a copy of its window in LDS: the packed band columns
-//    [cq, r1] (all 2b stored rows), which hold every entry the hop reads or
-//    writes. The Householder vector is formed by wave 0 alone (L <= 64 = one
-//    wave64), with FP64 norms.
+//  * Each hop works on its window in LDS: the packed band columns [cq, r1]
+//    (all 2b stored rows) hold every entry the hop reads or writes. The window
+//    slides with the sweep: consecutive hops share b columns, so a hop loads
+//    only the b new columns and writes back only the b columns the sweep has
+//    finished with (a ring of 2b column slots). The Householder vector is formed
+//    by wave 0 alone (L <= 64 = one wave64), with FP64 norms.
 //  * Dependency: hop (s, cq, r1) touches packed columns [cq, r1] only, and the
 //    previous sweep's hops touch columns [cq'', r1''] with cq'' increasing, so
 //    sweep s may start the hop once sweep s-1 has completed its hop with
@@ -46,17 +48,40 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                                                               int *prog, int *err) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int ldb = 2 * b;       // packed band leading dimension
-    const int WLD = 2 * b + 2;   // LDS window leading dimension (padded against bank conflicts)
-    T *win = reinterpret_cast<T *>(smem_raw);          // [2b cols][WLD]
-    T *sw = win + (size_t)2 * b * WLD;                 // [b]   reflector w
+    const int WLD = 2 * b + 2;   // LDS leading dimension of a window column (padded)
+    const int NSLOT = 2 * b;     // ring of window columns: matrix column j lives in slot j % NSLOT
+    T *win = reinterpret_cast<T *>(smem_raw);          // [NSLOT][WLD]
+    T *sw = win + (size_t)NSLOT * WLD;                 // [b]   reflector w
     T *sp = sw + b;                                    // [3b]  p, then y
     __shared__ double s_beta;
     __shared__ double s_kappa;
     __shared__ int s_go;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // window element A(i, j) (i >= j, both in the current window)
+    auto W_ = [&](int i, int j) -> T & { return win[(j % NSLOT) * WLD + (i - j)]; };
+    // copy packed columns [c0, c1] between global memory and their ring slots
+    auto load_cols = [&](int c0, int c1) {
+        if (c1 < c0) return;
+        const int cnt = (c1 - c0 + 1) * ldb;
+        const T *src = Bp + (i64)c0 * ldb;
+        for (int idx = tid; idx < cnt; idx += kBCThreads) {
+            const int jj = idx / ldb, d = idx - jj * ldb;
+            win[((c0 + jj) % NSLOT) * WLD + d] = src[idx];
+        }
+    };
+    auto store_cols = [&](int c0, int c1) {
+        if (c1 < c0) return;
+        const int cnt = (c1 - c0 + 1) * ldb;
+        T *dst = Bp + (i64)c0 * ldb;
+        for (int idx = tid; idx < cnt; idx += kBCThreads) {
+            const int jj = idx / ldb, d = idx - jj * ldb;
+            dst[idx] = win[((c0 + jj) % NSLOT) * WLD + d];
+        }
+    };
 
     for (int s = blockIdx.x; s < n - 2; s += gridDim.x) {
+        int held_lo = 0, held_hi = -1; // window columns currently held in LDS
         for (int h = 0;; ++h) {
             const int cq = (h == 0) ? s : s + 1 + (h - 1) * b;
             const int r0 = (h == 0) ? s + 1 : cq + b;
@@ -69,7 +94,6 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 break;
             }
             const int hi = min(n - 1, r1 + b);  // last row touched
-            const int ncol = r1 - cq + 1;       // window columns [cq, r1]
 
             // ---- wait for sweep s-1 to be done with columns <= r1 (acquire)
             if (s > 0) {
@@ -92,17 +116,17 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
 
-            // ---- load the window (packed columns cq..r1, rows 0..2b-1)
-            const T *gsrc = Bp + (i64)cq * ldb;
-            for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
-                const int jj = idx / ldb, d = idx - jj * ldb;
-                win[jj * WLD + d] = gsrc[idx];
-            }
+            // ---- bring the window [cq, r1] into LDS: columns already held stay,
+            // the rest come from global memory (b columns per hop after the first)
+            if (held_hi < held_lo) load_cols(cq, r1);
+            else load_cols(max(cq, held_hi + 1), r1);
+            held_lo = cq;
+            held_hi = r1;
             __syncthreads();
 
             // ---- Householder vector on x = A[r0:r1, cq] (wave 0)
             if (wave == 0) {
-                const double x = (lane < L) ? (double)win[(r0 - cq) + lane] : 0.0;
+                const double x = (lane < L) ? (double)W_(r0 + lane, cq) : 0.0;
                 const double alpha = __shfl(x, 0, 64);
                 const double tail = wave_sum((lane >= 1) ? x * x : 0.0);
                 double wv = 0.0, beta = alpha;
@@ -125,7 +149,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 T acc = T(0);
                 for (int q = 0; q < L; ++q) {
                     const int k = r0 + q;
-                    const T a = (i >= k) ? win[(k - cq) * WLD + (i - k)] : win[(i - cq) * WLD + (k - i)];
+                    const T a = (i >= k) ? W_(i, k) : W_(k, i);
                     acc += a * sw[q];
                 }
                 sp[t] = T(2) * acc;
@@ -146,13 +170,14 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             __syncthreads();
 
             // ---- rank-2 update A(i,j) -= w_i y_j + y_i w_j on the window (i >= j, i <= hi)
+            const int ncol = r1 - cq + 1;
             for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
                 const int jj = idx / ldb, d = idx - jj * ldb;
                 const int j = cq + jj, i = j + d;
                 if (i > hi) continue;
                 const bool wi_nz = (i >= r0 && i <= r1), wj_nz = (j >= r0);
                 if (!wi_nz && !wj_nz) continue;
-                T *a = &win[jj * WLD + d];
+                T *a = &win[(j % NSLOT) * WLD + d];
                 if (jj == 0) {
                     // eliminated column: exact result H x = beta e_0
                     *a = (i == r0) ? (T)s_beta : T(0);
@@ -164,12 +189,12 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             }
             __syncthreads();
 
-            // ---- write back the window and the reflector
-            T *gdst = Bp + (i64)cq * ldb;
-            for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
-                const int jj = idx / ldb, d = idx - jj * ldb;
-                gdst[idx] = win[jj * WLD + d];
-            }
+            // ---- columns this sweep never touches again go back to global memory:
+            // [cq, next cq - 1] (the next hop starts at s+1 after hop 0, at cq+b after)
+            const int next_cq = (h == 0) ? s + 1 : cq + b;
+            const int keep_lo = min(next_cq, r1 + 1);
+            store_cols(cq, keep_lo - 1);
+            held_lo = keep_lo;
             T *ucol = U + (i64)(r0 - s - 1) + (i64)s * ldu;
             for (int t = tid; t < L; t += kBCThreads) ucol[t] = sw[t];
 
@@ -178,6 +203,8 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             __syncthreads();
             if (tid == 0) flag_store_release(prog + s, cq);
         }
+        // the rest of the last window
+        store_cols(held_lo, held_hi);
         __threadfence();
         __syncthreads();
         if (tid == 0) flag_store_release(prog + s, kDone);

@@ -83,8 +83,8 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
     double *sums = GT + (size_t)b * (b + 1);             // [2b] reduced payload
     double *tauv = sums + 2 * b;                         // [b]
     double *prow = tauv + b;                             // [b] pivot row
-    double *scal3 = prow + b;                            // [4] alpha/beta/tau/scal of current column
-    T *P = reinterpret_cast<T *>(scal3 + 4);             // [b][CHp] panel rows of this workgroup
+    double *red4 = prow + b;                             // [256] partial-sum combine buffer
+    T *P = reinterpret_cast<T *>(red4 + 256);            // [b][CHp] panel rows of this workgroup
 
     const int r0g = g * CH;                              // first global panel row owned
     const int chl = max(0, min(CH, a.rows - r0g));       // rows owned
@@ -138,24 +138,32 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
         // (2) grid-wide reduction
         if (!grid_barrier(a.counter, (unsigned)a.G * (unsigned)(c + 1), a.err)) alive = false;
 
-        if (tid < 2 * b) {
-            const bool is_dot = tid < ndot;
-            const bool is_gram = tid >= b && tid - b < ngram;
-            if (is_dot || is_gram) {
-                double s = 0.0;
-                const double *src = a.part + (size_t)buf * a.G * PS + tid;
-                int gg = 0;
-                for (; gg + 4 <= a.G; gg += 4) {
-                    const double v0 = src[(size_t)gg * PS], v1 = src[(size_t)(gg + 1) * PS];
-                    const double v2 = src[(size_t)(gg + 2) * PS], v3 = src[(size_t)(gg + 3) * PS];
+        // Sum the G partial vectors: 4 threads per payload entry, each taking every
+        // 4th workgroup with its loads issued back to back, then a fixed-order
+        // combine, so every workgroup obtains bit-identical sums.
+        {
+            const int q = tid & 63, part = tid >> 6;
+            double s = 0.0;
+            if (q < nq) {
+                const int slot = (q < ndot) ? q : b + (q - ndot);
+                const double *src = a.part + (size_t)buf * a.G * PS + slot;
+                int gg = part;
+                for (; gg + 12 < a.G; gg += 16) {
+                    const double v0 = src[(size_t)gg * PS], v1 = src[(size_t)(gg + 4) * PS];
+                    const double v2 = src[(size_t)(gg + 8) * PS], v3 = src[(size_t)(gg + 12) * PS];
                     s += v0;
                     s += v1;
                     s += v2;
                     s += v3;
                 }
-                for (; gg < a.G; ++gg) s += src[(size_t)gg * PS];
-                sums[tid] = s;
+                for (; gg < a.G; gg += 4) s += src[(size_t)gg * PS];
             }
+            red4[part * 64 + q] = s;
+        }
+        __syncthreads();
+        if (tid < nq) {
+            const int slot = (tid < ndot) ? tid : b + (tid - ndot);
+            sums[slot] = ((red4[tid] + red4[64 + tid]) + red4[128 + tid]) + red4[192 + tid];
         }
         if (has_col)
             for (int j = c + tid; j < b; j += kPQThreads) prow[j] = a.pivot[buf * b + j];
@@ -263,7 +271,7 @@ __global__ void zero_u32_kernel(unsigned *p, int n) {
 }
 
 template <typename T> size_t pq_lds_bytes(int b, int CH) {
-    return sizeof(double) * ((size_t)b * (b + 1) + 2 * b + b + b + 4) + sizeof(T) * (size_t)b * (CH + 1);
+    return sizeof(double) * ((size_t)b * (b + 1) + 2 * b + b + b + 256) + sizeof(T) * (size_t)b * (CH + 1);
 }
 
 } // namespace
