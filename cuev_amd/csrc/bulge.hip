@@ -43,10 +43,12 @@ namespace {
 constexpr int kBCThreads = 256;
 constexpr int kDone = 0x3fffffff; // prog value of a finished sweep
 
-template <typename T>
-__global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 ldu, int n, int b,
+template <typename T, int B>
+__global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 ldu, int n,
                                                               int *prog, int *err) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int b = B;
+    constexpr int NEW_PER_THREAD = 2 * B * B / kBCThreads; // b new columns of 2b rows per hop
     const int ldb = 2 * b;       // packed band leading dimension
     const int WLD = 2 * b + 2;   // LDS leading dimension of a window column (padded)
     const int NSLOT = 2 * b;     // ring of window columns: matrix column j lives in slot j % NSLOT
@@ -68,6 +70,19 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         for (int idx = tid; idx < cnt; idx += kBCThreads) {
             const int jj = idx / ldb, d = idx - jj * ldb;
             win[((c0 + jj) % NSLOT) * WLD + d] = src[idx];
+        }
+    };
+    // exactly b new columns: all loads issued before any LDS store (one round trip)
+    auto load_b_cols = [&](int c0) {
+        const T *src = Bp + (i64)c0 * ldb;
+        T v[NEW_PER_THREAD];
+#pragma unroll
+        for (int u = 0; u < NEW_PER_THREAD; ++u) v[u] = src[tid + u * kBCThreads];
+#pragma unroll
+        for (int u = 0; u < NEW_PER_THREAD; ++u) {
+            const int idx = tid + u * kBCThreads;
+            const int jj = idx / ldb, d = idx - jj * ldb;
+            win[((c0 + jj) % NSLOT) * WLD + d] = v[u];
         }
     };
     auto store_cols = [&](int c0, int c1) {
@@ -100,7 +115,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 if (tid == 0) {
                     long long it = 0;
                     int go = 1;
-                    while (flag_load_acquire(prog + (s - 1)) < r1) {
+                    while (flag_load_relaxed(prog + (s - 1)) < r1) {
                         __builtin_amdgcn_s_sleep(1);
                         if (++it > kSpinLimit) {
                             __hip_atomic_store(err, (int)DEV_WATCHDOG, __ATOMIC_RELAXED,
@@ -109,16 +124,17 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                             break;
                         }
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     s_go = go;
                 }
                 __syncthreads();
                 if (!s_go) return;
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
 
             // ---- bring the window [cq, r1] into LDS: columns already held stay,
             // the rest come from global memory (b columns per hop after the first)
             if (held_hi < held_lo) load_cols(cq, r1);
+            else if (r1 - held_hi == b) load_b_cols(held_hi + 1);
             else load_cols(max(cq, held_hi + 1), r1);
             held_lo = cq;
             held_hi = r1;
@@ -199,14 +215,12 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             for (int t = tid; t < L; t += kBCThreads) ucol[t] = sw[t];
 
             // ---- publish progress (release)
-            __threadfence();
-            __syncthreads();
+            wg_publish_fence();
             if (tid == 0) flag_store_release(prog + s, cq);
         }
         // the rest of the last window
         store_cols(held_lo, held_hi);
-        __threadfence();
-        __syncthreads();
+        wg_publish_fence();
         if (tid == 0) flag_store_release(prog + s, kDone);
     }
 }
@@ -226,32 +240,44 @@ template <typename T> size_t bc_lds_bytes(int b) {
 
 } // namespace
 
-template <typename T> int bc_chase(Workspace<T> &ws, T *Bp, T *d, T *e) {
-    const int n = ws.n, b = ws.b;
+template <typename T, int B> int launch_chase(Workspace<T> &ws, T *Bp) {
+    const int n = ws.n;
     hipStream_t s = ws.stream;
     const int nsweeps = n - 2;
-    if (nsweeps > 0) {
-        CUEV_TRY(fill_int(s, ws.prog, n, -1));
-        const size_t lds = bc_lds_bytes<T>(b);
-        CUEV_HIP(hipFuncSetAttribute((const void *)bc_chase_kernel<T>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        int per_cu = 0;
-        CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc_chase_kernel<T>,
-                                                              kBCThreads, lds));
-        const int resident = std::max(1, per_cu * device_cu_count());
-        // at most ~n/(2b) sweeps can be active at once; more workgroups would only spin
-        const int useful = (int)ceil_div(n, 2 * b) + 2;
-        const int grid = std::max(1, std::min({nsweeps, resident, useful}));
-        // algorithmic work: ~n^2/(2b) hops, each reading and writing a 2b x 2b window
-        // of the packed band (8 b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops
-        const double hops = (double)n * n / (2.0 * b);
-        prof::Scope sc(s, "bc.chase", hops * (6.0 * b * b + 16.0 * b * b), hops * 8.0 * b * b * sizeof(T));
-        hipLaunchKernelGGL(bc_chase_kernel<T>, dim3(grid), dim3(kBCThreads), lds, s, Bp, ws.U,
-                           ws.ldu, n, b, ws.prog, ws.err);
-        CUEV_LAUNCH_CHECK();
+    CUEV_TRY(fill_int(s, ws.prog, n, -1));
+    const size_t lds = bc_lds_bytes<T>(B);
+    CUEV_HIP(hipFuncSetAttribute((const void *)bc_chase_kernel<T, B>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc_chase_kernel<T, B>,
+                                                          kBCThreads, lds));
+    const int resident = std::max(1, per_cu * device_cu_count());
+    // at most ~n/(2b) sweeps can be active at once; more workgroups would only spin
+    const int useful = (int)ceil_div(n, 2 * B) + 2;
+    const int grid = std::max(1, std::min({nsweeps, resident, useful}));
+    // algorithmic work: ~n^2/(2b) hops, each reading and writing b columns of the
+    // packed band (2 * 2b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops
+    const double hops = (double)n * n / (2.0 * B);
+    prof::Scope sc(s, "bc.chase", hops * (6.0 * B * B + 16.0 * B * B), hops * 4.0 * B * B * sizeof(T));
+    hipLaunchKernelGGL((bc_chase_kernel<T, B>), dim3(grid), dim3(kBCThreads), lds, s, Bp, ws.U,
+                       ws.ldu, n, ws.prog, ws.err);
+    CUEV_LAUNCH_CHECK();
+    return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T> int bc_chase(Workspace<T> &ws, T *Bp, T *d, T *e) {
+    const int n = ws.n;
+    hipStream_t s = ws.stream;
+    if (n - 2 > 0) {
+        switch (ws.b) {
+        case 16: CUEV_TRY((launch_chase<T, 16>(ws, Bp))); break;
+        case 32: CUEV_TRY((launch_chase<T, 32>(ws, Bp))); break;
+        case 64: CUEV_TRY((launch_chase<T, 64>(ws, Bp))); break;
+        default: CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_chase: unsupported b=%d", ws.b);
+        }
     }
     hipLaunchKernelGGL(bc_extract_kernel<T>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, Bp,
-                       d, e, n, b);
+                       d, e, n, ws.b);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
 }

@@ -86,6 +86,26 @@ __device__ __forceinline__ unsigned counter_add_release(unsigned *p, unsigned v)
 __device__ __forceinline__ unsigned counter_load_acquire(const unsigned *p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Polling loads: relaxed, so a spin does not invalidate caches on every poll
+// (an agent-scope acquire load is a load followed by buffer_inv sc1 on gfx950).
+// The waiter issues one acquire fence after the condition holds.
+__device__ __forceinline__ int flag_load_relaxed(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned counter_load_relaxed(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup-wide publish/consume of global data across workgroups.
+// publish: every wave writes back its stores (release fence at agent scope:
+// buffer_wbl2 + waitcnt, no invalidate), then the workgroup barrier, after which
+// one thread may store the flag. consume: after one thread has observed the
+// flag with a relaxed poll, it issues the acquire fence (invalidates the CU's
+// L1 and the non-coherent L2 lines) before the barrier releases the others.
+__device__ __forceinline__ void wg_publish_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+}
 
 #endif // __HIPCC__
 

@@ -47,16 +47,15 @@ template <typename T> struct PQArgs {
     int *err;
 };
 
-// Grid barrier number `phase` (0-based): all G workgroups arrive, then all leave.
+// Grid barrier: all G workgroups arrive (counter reaches `target`), then all leave.
 __device__ __forceinline__ bool grid_barrier(unsigned *counter, unsigned target, int *err) {
-    __threadfence();
-    __syncthreads();
+    wg_publish_fence();
     __shared__ int s_ok;
     if (threadIdx.x == 0) {
         counter_add_release(counter, 1u);
         long long it = 0;
         int ok = 1;
-        while (counter_load_acquire(counter) < target) {
+        while (counter_load_relaxed(counter) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++it > kSpinLimit) {
                 __hip_atomic_store(err, (int)DEV_WATCHDOG, __ATOMIC_RELAXED,
@@ -65,10 +64,10 @@ __device__ __forceinline__ bool grid_barrier(unsigned *counter, unsigned target,
                 break;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return s_ok != 0;
 }
 
@@ -147,16 +146,17 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
             if (q < nq) {
                 const int slot = (q < ndot) ? q : b + (q - ndot);
                 const double *src = a.part + (size_t)buf * a.G * PS + slot;
-                int gg = part;
-                for (; gg + 12 < a.G; gg += 16) {
-                    const double v0 = src[(size_t)gg * PS], v1 = src[(size_t)(gg + 4) * PS];
-                    const double v2 = src[(size_t)(gg + 8) * PS], v3 = src[(size_t)(gg + 12) * PS];
-                    s += v0;
-                    s += v1;
-                    s += v2;
-                    s += v3;
+                // batches of 32 independent loads (one memory round trip per batch)
+                for (int g0 = part; g0 < a.G; g0 += 4 * 32) {
+                    double v[32];
+#pragma unroll
+                    for (int u = 0; u < 32; ++u) {
+                        const int gg = g0 + 4 * u;
+                        v[u] = (gg < a.G) ? src[(size_t)gg * PS] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 32; ++u) s += v[u];
                 }
-                for (; gg < a.G; gg += 4) s += src[(size_t)gg * PS];
             }
             red4[part * 64 + q] = s;
         }
