@@ -1,12 +1,431 @@
-// refine.hip — mixed-precision solve (FP32 stages + FP64 Ogita-Aishima refinement).
+// refine.hip — mixed-precision solve: FP32 two-stage reduction and back-transform,
+// host FP64 divide and conquer, FP64 Ogita-Aishima refinement on the GPU.
+// No counterpart in the reference (the reference runs every stage in T).
+//
+// Pipeline (double API, precision = CUEV_PRECISION_MIXED):
+//   1. A32 = (float) A; DBBR and bulge chasing in FP32 (FP32 MFMA, FP32
+//      accumulation; panel norms, dot products, tau and T stay FP64).
+//   2. Host dstedc in FP64 on the FP32-derived (d, e), overlapped with the FP32
+//      build of X32 = Q_s Q_b on the GPU; X = (double)(X32 (float)Q_d).
+//   3. Ogita-Aishima sweeps in FP64 (T. Ogita, K. Aishima, "Iterative
+//      refinement for symmetric eigenvalue decomposition", JJIAM 35, 2018,
+//      Algorithm 1):
+//        G = X^T X,  R = I - G,  S = X^T A X,  lam_i = s_ii / (1 - r_ii),
+//        delta = 2 (||S - diag(lam)||_2 + ||A||_2 ||R||_2),
+//        E_ij = (s_ij + lam_j r_ij) / (lam_j - lam_i)  if |lam_i - lam_j| > delta,
+//             = r_ij / 2                              otherwise,
+//        X <- X + X E.
+//      The 2-norms are bounded by Frobenius norms (a larger delta only moves
+//      more pairs to the r_ij/2 branch). Each sweep first measures the
+//      residual ||A X - X diag(lam)||_F / (n ||A||_F) and ||X^T X - I||_F / n
+//      from the products it needs anyway and stops once both are below the
+//      tolerance. Pairs inside a cluster (|lam_i - lam_j| <= delta) are
+//      orthogonalised but not rotated within the cluster; they are counted.
+//   All products run on the FP64 MFMA engine; the fused element-wise kernels
+//   below build lam, E and the norms.
 #include "refine.hpp"
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "backtransform.hpp"
+#include "bulge.hpp"
+#include "context.hpp"
+#include "dbbr.hpp"
+#include "gemm.hpp"
+#include "lapack.hpp"
+#include "profile.hpp"
+#include "util.hpp"
+
 namespace cuev {
+namespace {
+
+constexpr int kRedBlocks = 1024; // fixed partial count: deterministic reductions
+
+// lam_i = S_ii / G_ii
+__global__ void oa_lambda_kernel(const double *S, const double *G, i64 ld, int n, double *lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lam[i] = S[i + (i64)i * ld] / G[i + (i64)i * ld];
+}
+
+// block partial sums of squares:
+//  part[0*kRedBlocks + b] : ||S - diag(lam)||_F^2
+//  part[1*kRedBlocks + b] : ||G - I||_F^2            (= ||R||_F^2)
+//  part[2*kRedBlocks + b] : ||AX - X diag(lam)||_F^2
+__global__ __launch_bounds__(256) void oa_norms_kernel(const double *S, const double *G,
+                                                       const double *AX, const double *X, i64 ld,
+                                                       int n, const double *lam, double *part) {
+    __shared__ double red[3][256];
+    double a = 0, g = 0, r = 0;
+    const i64 total = (i64)n * n;
+    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (i64)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        const i64 o = i + (i64)j * ld;
+        const double s = S[o] - (i == j ? lam[i] : 0.0);
+        const double q = G[o] - (i == j ? 1.0 : 0.0);
+        const double t = AX[o] - X[o] * lam[j];
+        a += s * s;
+        g += q * q;
+        r += t * t;
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = g;
+    red[2][threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[threadIdx.x * kRedBlocks + blockIdx.x] = red[threadIdx.x][0];
+}
+
+// ||A||_F^2 of the symmetric matrix stored in the lower triangle
+__global__ __launch_bounds__(256) void sym_frob_kernel(const double *A, i64 ld, int n, double *part) {
+    __shared__ double red[256];
+    double a = 0;
+    const i64 total = (i64)n * n;
+    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (i64)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        if (i < j) continue;
+        const double v = A[i + (i64)j * ld];
+        a += (i == j) ? v * v : 2.0 * v * v;
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// E (written over G) from S, G = X^T X, lam and delta; counts clustered pairs (i < j)
+__global__ __launch_bounds__(256) void oa_e_kernel(const double *S, double *G, i64 ld, int n,
+                                                   const double *lam, double delta,
+                                                   unsigned *nclustered) {
+    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const i64 o = i + (i64)j * ld;
+    const double r = (i == j ? 1.0 : 0.0) - G[o];
+    double e;
+    if (i == j) {
+        e = 0.5 * r;
+    } else {
+        const double gap = lam[j] - lam[i];
+        if (fabs(gap) > delta) {
+            e = (S[o] + lam[j] * r) / gap;
+        } else {
+            e = 0.5 * r;
+            if (i < j) atomicAdd(nclustered, 1u);
+        }
+    }
+    G[o] = e;
+}
+
+// Xp[:, j] = X[:, perm[j]]
+__global__ void permute_cols_kernel(const double *X, double *Xp, i64 ld, int n, const int *perm) {
+    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    Xp[i + (i64)j * ld] = X[i + (i64)perm[j] * ld];
+}
+
+struct MixedBuffers {
+    int n = 0, device = -1;
+    double *T1 = nullptr, *S = nullptr, *G = nullptr; // n x n FP64
+    double *lam = nullptr;                            // n
+    double *part = nullptr;                           // 3 * kRedBlocks
+    unsigned *count = nullptr;
+    int *perm = nullptr;                              // n
+    double *h_d = nullptr, *h_e = nullptr, *h_z = nullptr; // pinned
+    double *h_work = nullptr;
+    int64_t *h_iwork = nullptr;
+    i64 lwork = 0, liwork = 0;
+    void *pool = nullptr;
+};
+
+void mixed_free(MixedBuffers *m) {
+    if (!m) return;
+    if (m->pool) (void)hipFree(m->pool);
+    if (m->h_d) (void)hipHostFree(m->h_d);
+    if (m->h_e) (void)hipHostFree(m->h_e);
+    if (m->h_z) (void)hipHostFree(m->h_z);
+    std::free(m->h_work);
+    std::free(m->h_iwork);
+    delete m;
+}
+
+int mixed_alloc(MixedBuffers *&out, int n) {
+    auto *m = new MixedBuffers();
+    m->n = n;
+    (void)hipGetDevice(&m->device);
+    const size_t nn = (size_t)n * n;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t bytes = 3 * al(nn * 8) + al(n * 8) + al(3 * kRedBlocks * 8) + al(64) + al(n * 4);
+    if (hipMalloc(&m->pool, bytes) != hipSuccess) {
+        m->pool = nullptr;
+        mixed_free(m);
+        CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "mixed: hipMalloc(%zu) failed", bytes);
+    }
+    auto *p = static_cast<unsigned char *>(m->pool);
+    m->T1 = reinterpret_cast<double *>(p);
+    p += al(nn * 8);
+    m->S = reinterpret_cast<double *>(p);
+    p += al(nn * 8);
+    m->G = reinterpret_cast<double *>(p);
+    p += al(nn * 8);
+    m->lam = reinterpret_cast<double *>(p);
+    p += al(n * 8);
+    m->part = reinterpret_cast<double *>(p);
+    p += al(3 * kRedBlocks * 8);
+    m->count = reinterpret_cast<unsigned *>(p);
+    p += al(64);
+    m->perm = reinterpret_cast<int *>(p);
+    if (hipHostMalloc(&m->h_d, n * 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&m->h_e, n * 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&m->h_z, nn * 8, hipHostMallocDefault) != hipSuccess) {
+        mixed_free(m);
+        CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "mixed: pinned host allocation failed");
+    }
+    m->lwork = stedc_lwork(n);
+    m->liwork = stedc_liwork(n);
+    m->h_work = static_cast<double *>(std::malloc(8 * (size_t)m->lwork));
+    m->h_iwork = static_cast<int64_t *>(std::malloc(8 * (size_t)m->liwork));
+    if (!m->h_work || !m->h_iwork) {
+        mixed_free(m);
+        CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "mixed: host allocation failed");
+    }
+    out = m;
+    return CUEV_STATUS_SUCCESS;
+}
+
+std::mutex g_mix_mu;
+MixedBuffers *g_mix_idle[64] = {};
+
+int mixed_acquire(MixedBuffers *&m, int n) {
+    int dev = 0;
+    CUEV_HIP(hipGetDevice(&dev));
+    MixedBuffers *victim = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mix_mu);
+        MixedBuffers *&slot = g_mix_idle[dev & 63];
+        if (slot && slot->n == n && slot->device == dev) {
+            m = slot;
+            slot = nullptr;
+            return CUEV_STATUS_SUCCESS;
+        }
+        victim = slot;
+        slot = nullptr;
+    }
+    if (victim) {
+        (void)hipDeviceSynchronize();
+        mixed_free(victim);
+    }
+    return mixed_alloc(m, n);
+}
+
+void mixed_release(MixedBuffers *m) {
+    MixedBuffers *victim = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mix_mu);
+        MixedBuffers *&slot = g_mix_idle[m->device & 63];
+        victim = slot;
+        slot = m;
+    }
+    if (victim) mixed_free(victim);
+}
+
+using clk = std::chrono::steady_clock;
+double ms_between(clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+double sum_parts(const std::vector<double> &h, int which) {
+    double s = 0;
+    for (int b = 0; b < kRedBlocks; ++b) s += h[(size_t)which * kRedBlocks + b];
+    return s;
+}
+
+int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *eval, double *evec,
+              cuev_solve_timer *timer, const cuev_options &opt) {
+    Workspace<float> &ws = cf.ws;
+    hipStream_t s = ws.stream;
+    Scratch gs32{ws.gscratch, ws.gscratch_bytes};
+    const auto wall0 = clk::now();
+    hipEvent_t *ev = cf.ev;
+
+    // 1. FP32 reduction (A32 lives in ws.M until the band is packed)
+    CUEV_HIP(hipEventRecord(ev[0], s));
+    CUEV_TRY((convert2d<double, float>(s, A, n, ws.M, ws.ld, n, n)));
+    CUEV_TRY(dbbr_reduce<float>(ws, ws.M, ws.B));
+    CUEV_HIP(hipEventRecord(ev[1], s));
+    CUEV_TRY(bc_chase<float>(ws, ws.B, ws.d, ws.e));
+    CUEV_HIP(hipEventRecord(ev[2], s));
+    CUEV_HIP(hipMemcpyAsync(cf.h_d, ws.d, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipMemcpyAsync(cf.h_e, ws.e, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipEventRecord(ev[3], s));
+    // 2. FP32 X32 = Q_s Q_b on the GPU while the host runs the FP64 D&C
+    CUEV_TRY(fill2d<float>(s, ws.M, ws.ld, n, n, 0.f, 1.f));
+    CUEV_TRY(bc_back<float>(ws, ws.U, ws.M, ws.ld));
+    CUEV_TRY(sbr_back<float>(ws, ws.Y, ws.W, ws.M, ws.ld));
+    CUEV_HIP(hipEventRecord(ev[4], s));
+    CUEV_HIP(hipEventSynchronize(ev[3]));
+    for (int i = 0; i < n; ++i) {
+        mb.h_d[i] = cf.h_d[i];
+        mb.h_e[i] = (i < n - 1) ? cf.h_e[i] : 0.0;
+        if (!std::isfinite(mb.h_d[i]) || !std::isfinite(mb.h_e[i]))
+            CUEV_FAIL(CUEV_STATUS_NOT_FINITE, "mixed: non-finite tridiagonal entry at %d", i);
+    }
+    const auto dc0 = clk::now();
+    CUEV_TRY(stedc<double>(n, mb.h_d, mb.h_e, mb.h_z, n, mb.h_work, mb.lwork, mb.h_iwork, mb.liwork));
+    const auto dc1 = clk::now();
+    CUEV_HIP(hipEventRecord(ev[5], s));
+    // X = (double)(X32 * (float)Q_d): Q_d through S (FP64) into W (FP32, free after sbr_back)
+    CUEV_HIP(hipMemcpyAsync(mb.S, mb.h_z, 8 * (size_t)n * n, hipMemcpyHostToDevice, s));
+    CUEV_HIP(hipMemcpyAsync(mb.lam, mb.h_d, 8 * (size_t)n, hipMemcpyHostToDevice, s));
+    CUEV_TRY((convert2d<double, float>(s, mb.S, n, ws.W, ws.ld, n, n)));
+    {
+        prof::Scope sc(s, "mixed.bt_gemm32", 2.0 * (double)n * n * n);
+        CUEV_TRY(gemm<float>(s, 0, 0, n, n, n, 1.f, ws.M, ws.ld, ws.W, ws.ld, 0.f, ws.Y, ws.ld, gs32));
+    }
+    CUEV_TRY((convert2d<float, double>(s, ws.Y, ws.ld, evec, n, n, n)));
+    CUEV_HIP(hipEventRecord(ev[6], s));
+
+    // 3. FP64 refinement (X in evec, A original in A)
+    // FP64 products share the FP32 workspace's split-K scratch (the engine limits
+    // the split count to what fits)
+    Scratch gs64{ws.gscratch, ws.gscratch_bytes};
+    const int max_sweeps = opt.refine_sweeps > 0 ? opt.refine_sweeps : 3;
+    const double tol = opt.refine_tol > 0 ? opt.refine_tol : 2e-14;
+    std::vector<double> hp(3 * (size_t)kRedBlocks);
+    hipLaunchKernelGGL(sym_frob_kernel, dim3(kRedBlocks), dim3(256), 0, s, A, (i64)n, n, mb.part);
+    CUEV_LAUNCH_CHECK();
+    CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * (size_t)kRedBlocks, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    const double afro = std::sqrt(sum_parts(hp, 0));
+    const double anorm = afro > 0 ? afro : 1.0;
+    double *X = evec, *Xn = mb.T1;
+    int sweeps = 0;
+    double res = 0, orth = 0;
+    unsigned nclus = 0;
+    const auto ref0 = clk::now();
+    for (int it = 0;; ++it) {
+        prof::Scope sc(s, "mixed.refine_sweep", 8.0 * (double)n * n * n);
+        // AX = A X (A lower), S = X^T AX, G = X^T X
+        CUEV_TRY(symm_lower<double>(s, n, n, 1.0, A, n, X, n, 0.0, mb.T1, n, gs64));
+        CUEV_TRY(gemm<double>(s, 1, 0, n, n, n, 1.0, X, n, mb.T1, n, 0.0, mb.S, n, gs64));
+        CUEV_TRY(gemm<double>(s, 1, 0, n, n, n, 1.0, X, n, X, n, 0.0, mb.G, n, gs64));
+        hipLaunchKernelGGL(oa_lambda_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, mb.S,
+                           mb.G, (i64)n, n, mb.lam);
+        CUEV_LAUNCH_CHECK();
+        hipLaunchKernelGGL(oa_norms_kernel, dim3(kRedBlocks), dim3(256), 0, s, mb.S, mb.G, mb.T1, X,
+                           (i64)n, n, mb.lam, mb.part);
+        CUEV_LAUNCH_CHECK();
+        CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * hp.size(), hipMemcpyDeviceToHost, s));
+        CUEV_HIP(hipStreamSynchronize(s));
+        const double s_off = std::sqrt(sum_parts(hp, 0)), r_fro = std::sqrt(sum_parts(hp, 1));
+        res = std::sqrt(sum_parts(hp, 2)) / ((double)n * anorm);
+        orth = r_fro / n;
+        if ((res <= tol && orth <= tol) || it >= max_sweeps) break;
+        // E over G, then X <- X + X E
+        std::vector<double> hl(n);
+        CUEV_HIP(hipMemcpyAsync(hl.data(), mb.lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+        CUEV_HIP(hipStreamSynchronize(s));
+        double lmax = 0;
+        for (double v : hl) lmax = std::max(lmax, std::fabs(v));
+        const double delta = 2.0 * (s_off + lmax * r_fro);
+        CUEV_TRY(fill_int(s, reinterpret_cast<int *>(mb.count), 1, 0));
+        hipLaunchKernelGGL(oa_e_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
+                           mb.S, mb.G, (i64)n, n, mb.lam, delta, mb.count);
+        CUEV_LAUNCH_CHECK();
+        CUEV_TRY(copy2d<double>(s, X, n, Xn, n, n, n));
+        CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, X, n, mb.G, n, 1.0, Xn, n, gs64));
+        CUEV_HIP(hipMemcpyAsync(&nclus, mb.count, 4, hipMemcpyDeviceToHost, s));
+        std::swap(X, Xn);
+        ++sweeps;
+    }
+    const auto ref1 = clk::now();
+    // eigenvalues ascending; columns follow
+    std::vector<double> hl(n);
+    CUEV_HIP(hipMemcpyAsync(hl.data(), mb.lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return hl[a] < hl[b]; });
+    bool sorted = true;
+    for (int i = 0; i < n; ++i) sorted &= perm[i] == i;
+    std::vector<double> hs(n);
+    for (int i = 0; i < n; ++i) hs[i] = hl[perm[i]];
+    CUEV_HIP(hipMemcpyAsync(eval, hs.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s));
+    if (!sorted || X != evec) {
+        double *dst = (X != evec) ? evec : Xn;
+        if (!sorted) {
+            CUEV_HIP(hipMemcpyAsync(mb.perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(permute_cols_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256),
+                               0, s, X, dst, (i64)n, n, mb.perm);
+            CUEV_LAUNCH_CHECK();
+        } else {
+            CUEV_TRY(copy2d<double>(s, X, n, dst, n, n, n));
+        }
+        if (dst != evec) CUEV_TRY(copy2d<double>(s, dst, n, evec, n, n, n));
+    }
+    CUEV_HIP(hipStreamSynchronize(s));
+    CUEV_TRY(workspace_check_error(ws));
+    const auto wall1 = clk::now();
+    if (timer) {
+        std::memset(timer, 0, sizeof(*timer));
+        auto em = [&](int a, int b) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+            return ms;
+        };
+        timer->dbbr_ms = em(0, 1);
+        timer->bc_ms = em(1, 2);
+        timer->dc_ms = (float)ms_between(dc0, dc1);
+        timer->overlap_ms = em(3, 4);
+        timer->bt_copyout_ms = em(5, 6);
+        timer->bt_ms = (float)ms_between(dc1, ref0);
+        timer->refine_ms = (float)ms_between(ref0, ref1);
+        timer->total_ms = (float)ms_between(wall0, wall1);
+        timer->refine_sweeps = sweeps;
+        timer->n_cluster_cols = (int)nclus;
+        timer->refine_resid = (float)res;
+        timer->refine_orth = (float)orth;
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
+} // namespace
 
 int solve_mixed(double *A, int n, double *eval, double *evec, hipStream_t s, cuev_solve_timer *timer,
                 const cuev_options &opt, int b, int k) {
-    (void)A; (void)n; (void)eval; (void)evec; (void)s; (void)timer; (void)opt; (void)b; (void)k;
-    CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "mixed precision form not built yet");
+    Context<float> *cf = nullptr;
+    CUEV_TRY(context_acquire<float>(cf, n, b, k, s));
+    MixedBuffers *mb = nullptr;
+    int st = mixed_acquire(mb, n);
+    if (st) {
+        context_release<float>(cf);
+        return st;
+    }
+    st = run_mixed(*cf, *mb, A, n, eval, evec, timer, opt);
+    if (st == CUEV_STATUS_SUCCESS) {
+        context_release<float>(cf);
+        mixed_release(mb);
+    } else {
+        (void)hipStreamSynchronize(s);
+        context_destroy<float>(cf);
+        mixed_free(mb);
+    }
+    return st;
 }
 
 } // namespace cuev

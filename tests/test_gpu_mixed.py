@@ -1,0 +1,53 @@
+"""Mixed form: FP32 two-stage reduction + FP64 Ogita-Aishima refinement must reach the
+FP64 tolerances (eigenvalues 1e-12 ||A||_2, residual and orthogonality 1e-13)."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import cuev_amd as cv
+from helpers import eig_metrics
+
+pytestmark = pytest.mark.gpu
+
+
+def run(torch, A, **kw):
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    lam, V, t = cv.symm_eig_solve(dA, precision="mixed", timer=True, **kw)
+    torch.cuda.synchronize()
+    return lam.cpu().numpy(), V.cpu().numpy(), t
+
+
+@pytest.mark.parametrize("n", [300, 1024, 2500])
+def test_mixed_random_symmetric(torch_dev, n):
+    rng = np.random.default_rng(n + 1)
+    G = rng.standard_normal((n, n))
+    A = (G + G.T) / 2
+    lam, V, t = run(torch_dev, A)
+    lam_ref = sl.eigh(A, eigvals_only=True, driver="evd")
+    ev, res, orth = eig_metrics(A, lam, V, lam_ref)
+    assert t["refine_sweeps"] >= 1
+    assert ev < 1e-12 and res < 1e-13 and orth < 1e-13, (ev, res, orth, t)
+
+
+def test_mixed_geometric_with_cluster(torch_dev):
+    """Scaled-down mixed16k: geometric spectrum 1..1e-6, random signs, 10% in a 1e-9 cluster."""
+    torch = torch_dev
+    from cuev_amd import workloads
+    n = 2048
+    Q = workloads.orthogonal_q(torch, n, 1, "cuda", torch.float64)
+    lam_x = workloads.lam_geometric_cluster(torch, n, 103, "cuda")
+    A = workloads.sandwich(torch, Q, lam_x)
+    lam, V, t = cv.symm_eig_solve(A.clone(), precision="mixed", timer=True)
+    torch.cuda.synchronize()
+    An = A.cpu().numpy()
+    lam_n, Vn = lam.cpu().numpy(), V.cpu().numpy()
+    ev, res, orth = eig_metrics(An, lam_n, Vn, lam_x.cpu().numpy())
+    assert ev < 1e-12 and res < 1e-13 and orth < 1e-13, (ev, res, orth, t)
+
+
+def test_mixed_rejected_for_float(torch_dev):
+    torch = torch_dev
+    A = torch.eye(8, dtype=torch.float32, device="cuda")
+    with pytest.raises(cv.CuevError) as e:
+        cv.symm_eig_solve(A, precision="mixed")
+    assert e.value.status == 8
