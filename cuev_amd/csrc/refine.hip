@@ -313,21 +313,23 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     CUEV_HIP(hipStreamSynchronize(s));
     const double afro = std::sqrt(sum_parts(hp, 0));
     const double anorm = afro > 0 ? afro : 1.0;
-    double *X = evec, *Xn = mb.T1;
+    // X alternates between evec and T1; the other buffer holds A X, then the next iterate
+    double *X = evec;
     int sweeps = 0;
     double res = 0, orth = 0;
     unsigned nclus = 0;
     const auto ref0 = clk::now();
     for (int it = 0;; ++it) {
         prof::Scope sc(s, "mixed.refine_sweep", 8.0 * (double)n * n * n);
-        // AX = A X (A lower), S = X^T AX, G = X^T X
-        CUEV_TRY(symm_lower<double>(s, n, n, 1.0, A, n, X, n, 0.0, mb.T1, n, gs64));
-        CUEV_TRY(gemm<double>(s, 1, 0, n, n, n, 1.0, X, n, mb.T1, n, 0.0, mb.S, n, gs64));
+        double *other = (X == evec) ? mb.T1 : evec;
+        // AX = A X (A lower) into the buffer not holding X, S = X^T AX, G = X^T X
+        CUEV_TRY(symm_lower<double>(s, n, n, 1.0, A, n, X, n, 0.0, other, n, gs64));
+        CUEV_TRY(gemm<double>(s, 1, 0, n, n, n, 1.0, X, n, other, n, 0.0, mb.S, n, gs64));
         CUEV_TRY(gemm<double>(s, 1, 0, n, n, n, 1.0, X, n, X, n, 0.0, mb.G, n, gs64));
         hipLaunchKernelGGL(oa_lambda_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, mb.S,
                            mb.G, (i64)n, n, mb.lam);
         CUEV_LAUNCH_CHECK();
-        hipLaunchKernelGGL(oa_norms_kernel, dim3(kRedBlocks), dim3(256), 0, s, mb.S, mb.G, mb.T1, X,
+        hipLaunchKernelGGL(oa_norms_kernel, dim3(kRedBlocks), dim3(256), 0, s, mb.S, mb.G, other, X,
                            (i64)n, n, mb.lam, mb.part);
         CUEV_LAUNCH_CHECK();
         CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * hp.size(), hipMemcpyDeviceToHost, s));
@@ -347,10 +349,11 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         hipLaunchKernelGGL(oa_e_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
                            mb.S, mb.G, (i64)n, n, mb.lam, delta, mb.count);
         CUEV_LAUNCH_CHECK();
-        CUEV_TRY(copy2d<double>(s, X, n, Xn, n, n, n));
-        CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, X, n, mb.G, n, 1.0, Xn, n, gs64));
+        // next iterate X + X E into `other` (A X is no longer needed)
+        CUEV_TRY(copy2d<double>(s, X, n, other, n, n, n));
+        CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, X, n, mb.G, n, 1.0, other, n, gs64));
         CUEV_HIP(hipMemcpyAsync(&nclus, mb.count, 4, hipMemcpyDeviceToHost, s));
-        std::swap(X, Xn);
+        X = other;
         ++sweeps;
     }
     const auto ref1 = clk::now();
@@ -367,7 +370,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     for (int i = 0; i < n; ++i) hs[i] = hl[perm[i]];
     CUEV_HIP(hipMemcpyAsync(eval, hs.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s));
     if (!sorted || X != evec) {
-        double *dst = (X != evec) ? evec : Xn;
+        double *dst = (X != evec) ? evec : mb.T1;
         if (!sorted) {
             CUEV_HIP(hipMemcpyAsync(mb.perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(permute_cols_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256),
