@@ -46,6 +46,7 @@ bool try_lib(const std::string &path, const char *prefix) {
     g_fns.dstedc = reinterpret_cast<LapackFns::dstedc_t>(d);
     g_fns.sstedc = reinterpret_cast<LapackFns::sstedc_t>(s);
     g_fns.dsyevd = reinterpret_cast<LapackFns::dsyevd_t>(sym("dsyevd_"));
+    g_fns.dsygvd = reinterpret_cast<LapackFns::dsygvd_t>(sym("dsygvd_"));
     g_fns.name = path;
     g_fns.handle = h;
     return true;

@@ -21,9 +21,13 @@ struct LapackFns {
     typedef void (*sstedc64_t)(const char *, const int64_t *, float *, float *, float *,
                                const int64_t *, float *, const int64_t *, int64_t *,
                                const int64_t *, int64_t *);
+    typedef void (*dsygvd_t)(const int *, const char *, const char *, const int *, double *,
+                             const int *, double *, const int *, double *, double *, const int *,
+                             int *, const int *, int *);
     dstedc_t dstedc = nullptr;
     sstedc_t sstedc = nullptr;
     dsyevd_t dsyevd = nullptr;
+    dsygvd_t dsygvd = nullptr;
     dstedc64_t dstedc64 = nullptr;
     sstedc64_t sstedc64 = nullptr;
     std::string name;

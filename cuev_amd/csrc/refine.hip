@@ -122,13 +122,39 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double *S, double *G, i
     } else {
         const double gap = lam[j] - lam[i];
         if (fabs(gap) > delta) {
-            e = (S[o] + lam[j] * r) / gap;
+            // S is symmetric in exact arithmetic; X^T (A X) is not quite, and its
+            // antisymmetric rounding divided by a small gap would feed straight into
+            // E + E^T, which must equal R for the orthogonality to converge
+            const double sij = 0.5 * (S[o] + S[j + (i64)i * ld]);
+            e = (sij + lam[j] * r) / gap;
         } else {
             e = 0.5 * r;
             if (i < j) atomicAdd(nclustered, 1u);
         }
     }
     G[o] = e;
+}
+
+// dst[:, t] = X[:, idx[t]] for t < m (n rows)
+__global__ void gather_cols_kernel(const double *X, double *dst, i64 ld, int n, const int *idx, int m) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (i64)n * m) return;
+    const int i = (int)(k % n), t = (int)(k / n);
+    dst[i + (i64)t * n] = X[i + (i64)idx[t] * ld];
+}
+// X[:, idx[t]] = src[:, t]
+__global__ void scatter_cols_kernel(const double *src, double *X, i64 ld, int n, const int *idx, int m) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (i64)n * m) return;
+    const int i = (int)(k % n), t = (int)(k / n);
+    X[i + (i64)idx[t] * ld] = src[i + (i64)t * n];
+}
+// Sj[a + b*m] = S[idx[a] + idx[b]*ld]
+__global__ void gather_sub_kernel(const double *S, double *Sj, i64 ld, const int *idx, int m) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (i64)m * m) return;
+    const int a = (int)(k % m), b = (int)(k / m);
+    Sj[k] = S[idx[a] + (i64)idx[b] * ld];
 }
 
 // Xp[:, j] = X[:, perm[j]]
@@ -255,6 +281,66 @@ double sum_parts(const std::vector<double> &h, int which) {
     return s;
 }
 
+// Rayleigh-Ritz on each cluster (columns idx, clusters [start[c], start[c+1])):
+// solve S_JJ w = theta G_JJ w (host dsygvd, so the rotated columns also come out
+// G-orthonormal within the cluster) and replace X_J by X_J W. S (= X^T A X) and
+// G (= X^T X) are overwritten; `out` is an n x n buffer that holds nothing live.
+int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, double *X, double *out, int n,
+                  const std::vector<int> &idx, const std::vector<int> &start, Scratch gs) {
+    const int m = (int)idx.size();
+    const LapackFns *L = lapack();
+    if (!L || !L->dsygvd)
+        CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE, "mixed: host dsygvd needed for clustered eigenvalues");
+    CUEV_HIP(hipMemcpyAsync(mb.perm, idx.data(), 4 * (size_t)m, hipMemcpyHostToDevice, s));
+    // S_JJ, then G_JJ, as m x m matrices through `out` to the host
+    const unsigned gsub = (unsigned)ceil_div((i64)m * m, 256);
+    double *Sh = mb.h_z; // pinned, n*n >= m*m doubles
+    std::vector<double> gvec((size_t)m * m);
+    const double *Gh = gvec.data();
+    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.S, out, (i64)n, mb.perm, m);
+    CUEV_LAUNCH_CHECK();
+    CUEV_HIP(hipMemcpyAsync(Sh, out, 8 * (size_t)m * m, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.G, out, (i64)n, mb.perm, m);
+    CUEV_LAUNCH_CHECK();
+    CUEV_HIP(hipMemcpyAsync(gvec.data(), out, 8 * (size_t)m * m, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    std::vector<double> W((size_t)m * m, 0.0), sb, gb, wv, work;
+    std::vector<int> iwork;
+    for (size_t c = 0; c + 1 < start.size(); ++c) {
+        const int a = start[c], sz = start[c + 1] - start[c];
+        sb.assign((size_t)sz * sz, 0.0);
+        gb.assign((size_t)sz * sz, 0.0);
+        for (int q = 0; q < sz; ++q)
+            for (int p = 0; p < sz; ++p) {
+                sb[p + (size_t)q * sz] = Sh[(a + p) + (size_t)(a + q) * m];
+                gb[p + (size_t)q * sz] = Gh[(a + p) + (size_t)(a + q) * m];
+            }
+        wv.assign(sz, 0.0);
+        const int itype = 1;
+        const char jobz = 'V', uplo = 'L';
+        int nn = sz, lda = sz, info = 0;
+        int lwork = 1 + 6 * sz + 2 * sz * sz, liwork = 3 + 5 * sz;
+        work.resize(lwork);
+        iwork.resize(liwork);
+        L->dsygvd(&itype, &jobz, &uplo, &nn, sb.data(), &lda, gb.data(), &lda, wv.data(), work.data(),
+                  &lwork, iwork.data(), &liwork, &info);
+        if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "mixed: dsygvd on a cluster returned %d", info);
+        for (int q = 0; q < sz; ++q)
+            for (int p = 0; p < sz; ++p) W[(a + p) + (size_t)(a + q) * m] = sb[p + (size_t)q * sz];
+    }
+    // X_J into S (n x m), W into G (m x m), X_J W into out, scattered back into X
+    CUEV_HIP(hipMemcpyAsync(mb.G, W.data(), 8 * (size_t)m * m, hipMemcpyHostToDevice, s));
+    const unsigned gcol = (unsigned)ceil_div((i64)n * m, 256);
+    hipLaunchKernelGGL(gather_cols_kernel, dim3(gcol), dim3(256), 0, s, X, mb.S, (i64)n, n, mb.perm, m);
+    CUEV_LAUNCH_CHECK();
+    CUEV_TRY(gemm<double>(s, 0, 0, n, m, m, 1.0, mb.S, n, mb.G, m, 0.0, out, n, gs));
+    hipLaunchKernelGGL(scatter_cols_kernel, dim3(gcol), dim3(256), 0, s, out, X, (i64)n, n, mb.perm, m);
+    CUEV_LAUNCH_CHECK();
+    CUEV_HIP(hipStreamSynchronize(s));
+    return CUEV_STATUS_SUCCESS;
+}
+
 int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *eval, double *evec,
               cuev_solve_timer *timer, const cuev_options &opt) {
     Workspace<float> &ws = cf.ws;
@@ -315,11 +401,14 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     const double anorm = afro > 0 ? afro : 1.0;
     // X alternates between evec and T1; the other buffer holds A X, then the next iterate
     double *X = evec;
-    int sweeps = 0;
+    // sweeps counts every evaluation-and-update (Newton and Rayleigh-Ritz); the
+    // limit applies to the Newton steps
+    int sweeps = 0, oa_sweeps = 0, nclus_cols = 0;
+    bool did_rr = false;
     double res = 0, orth = 0;
     unsigned nclus = 0;
     const auto ref0 = clk::now();
-    for (int it = 0;; ++it) {
+    for (;;) {
         prof::Scope sc(s, "mixed.refine_sweep", 8.0 * (double)n * n * n);
         double *other = (X == evec) ? mb.T1 : evec;
         // AX = A X (A lower) into the buffer not holding X, S = X^T AX, G = X^T X
@@ -337,14 +426,46 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         const double s_off = std::sqrt(sum_parts(hp, 0)), r_fro = std::sqrt(sum_parts(hp, 1));
         res = std::sqrt(sum_parts(hp, 2)) / ((double)n * anorm);
         orth = r_fro / n;
-        if ((res <= tol && orth <= tol) || it >= max_sweeps) break;
-        // E over G, then X <- X + X E
+        if ((res <= tol && orth <= tol) || oa_sweeps >= max_sweeps) break;
         std::vector<double> hl(n);
         CUEV_HIP(hipMemcpyAsync(hl.data(), mb.lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
         CUEV_HIP(hipStreamSynchronize(s));
         double lmax = 0;
         for (double v : hl) lmax = std::max(lmax, std::fabs(v));
         const double delta = 2.0 * (s_off + lmax * r_fro);
+
+        // Clusters (runs of the sorted lam with neighbours closer than delta): the
+        // Newton step only orthogonalises inside a cluster, so the vectors there stay
+        // at the FP32 start's accuracy. Before each Newton step a Rayleigh-Ritz step
+        // on every cluster resolves them (the cluster treatment of Ogita and
+        // Aishima's refinement for clustered eigenvalues, JJIAM 36, 2019); S and G
+        // are then recomputed for the rotated basis.
+        if (!did_rr) {
+            std::vector<int> ord(n);
+            std::iota(ord.begin(), ord.end(), 0);
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hl[a] < hl[b]; });
+            std::vector<int> idx, start;
+            for (int i = 0; i < n;) {
+                int j = i + 1;
+                while (j < n && hl[ord[j]] - hl[ord[j - 1]] <= delta) ++j;
+                if (j - i >= 2) {
+                    start.push_back((int)idx.size());
+                    for (int t = i; t < j; ++t) idx.push_back(ord[t]);
+                }
+                i = j;
+            }
+            const int m = (int)idx.size();
+            if (m > 0) {
+                start.push_back(m);
+                CUEV_TRY(rayleigh_ritz(mb, s, X, other, n, idx, start, gs64));
+                nclus_cols = std::max(nclus_cols, m);
+                did_rr = true;
+                ++sweeps;
+                continue;
+            }
+        }
+        did_rr = false;
+        // E over G, then X <- X + X E
         CUEV_TRY(fill_int(s, reinterpret_cast<int *>(mb.count), 1, 0));
         hipLaunchKernelGGL(oa_e_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
                            mb.S, mb.G, (i64)n, n, mb.lam, delta, mb.count);
@@ -355,6 +476,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         CUEV_HIP(hipMemcpyAsync(&nclus, mb.count, 4, hipMemcpyDeviceToHost, s));
         X = other;
         ++sweeps;
+        ++oa_sweeps;
     }
     const auto ref1 = clk::now();
     // eigenvalues ascending; columns follow
@@ -400,7 +522,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         timer->refine_ms = (float)ms_between(ref0, ref1);
         timer->total_ms = (float)ms_between(wall0, wall1);
         timer->refine_sweeps = sweeps;
-        timer->n_cluster_cols = (int)nclus;
+        timer->n_cluster_cols = nclus_cols > 0 ? nclus_cols : (int)nclus;
         timer->refine_resid = (float)res;
         timer->refine_orth = (float)orth;
     }
