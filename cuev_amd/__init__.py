@@ -31,6 +31,7 @@ STATUS = {
 BUF = {"Y": 0, "W": 1, "B": 2, "U": 3, "D": 4, "E": 5, "TRI": 6, "TAU": 7, "M": 8}
 PRECISION = {"native": 0, "mixed": 1}
 BT_MODE = {"auto": 0, "direct": 1, "reordered": 2}
+DC_MODE = {"auto": 0, "host": 1, "device": 2}
 
 
 class CuevError(RuntimeError):
@@ -55,6 +56,7 @@ class SolveTimer(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("bandwidth", ctypes.c_int), ("block", ctypes.c_int), ("precision", ctypes.c_int),
                 ("refine_sweeps", ctypes.c_int), ("bt_mode", ctypes.c_int),
+                ("dc_mode", ctypes.c_int), ("dc_leaf", ctypes.c_int),
                 ("refine_tol", ctypes.c_double)]
 
 
@@ -103,6 +105,7 @@ def _declare(L):
     L.cuev_handle_create.argtypes = [ctypes.POINTER(P), I, I, I, I, P]
     L.cuev_handle_destroy.argtypes = [P]
     L.cuev_handle_buffer.argtypes = [P, I, ctypes.POINTER(P), ctypes.POINTER(I64)]
+    L.cuev_handle_set_dc.argtypes = [P, I, I]
 
 
 def _check(st):
@@ -154,7 +157,8 @@ def _ptr(t):
 
 
 def symm_eig_solve(A, *, overwrite=False, bandwidth=0, block=0, precision="native",
-                   refine_sweeps=0, bt_mode="auto", refine_tol=0.0, timer=False, stream=None):
+                   refine_sweeps=0, bt_mode="auto", dc_mode="auto", dc_leaf=0, refine_tol=0.0,
+                   timer=False, stream=None):
     """Eigen-decomposition of the symmetric (n, n) CUDA tensor ``A`` (lower triangle read).
 
     Returns ``(eval, V)`` with ascending eigenvalues and eigenvectors in the columns
@@ -179,6 +183,8 @@ def symm_eig_solve(A, *, overwrite=False, bandwidth=0, block=0, precision="nativ
     o.precision = PRECISION[precision]
     o.refine_sweeps = int(refine_sweeps)
     o.bt_mode = BT_MODE[bt_mode]
+    o.dc_mode = DC_MODE[dc_mode]
+    o.dc_leaf = int(dc_leaf)
     o.refine_tol = float(refine_tol)
     fn = getattr(lib(), f"cuev_{pre}symm_eig_solve_ex")
     _check(fn(_ptr(work), n, _ptr(ev), _ptr(vec), _stream_ptr(stream), ctypes.byref(t), ctypes.byref(o)))
@@ -261,6 +267,10 @@ class Handle:
         d, _ = self.buffer("D")
         e, _ = self.buffer("E")
         _check(self._f("bc_chase")(self.h, ctypes.c_void_p(B), ctypes.c_void_p(d), ctypes.c_void_p(e)))
+
+    def set_dc(self, mode="host", leaf=0):
+        """Tridiagonal stage: host LAPACK ?stedc ("host") or the device-resident D&C."""
+        _check(lib().cuev_handle_set_dc(self.h, DC_MODE[mode], int(leaf)))
 
     def tridi_dc(self, d, e, ev, vec):
         _check(self._f("tridi_dc")(self.h, _ptr(d), _ptr(e), _ptr(ev), _ptr(vec)))

@@ -48,6 +48,11 @@ int solve_entry(T *A, int n, T *eval, T *evec, hipStream_t s, cuev_solve_timer *
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_solve: null device pointer");
     if (o.bt_mode < CUEV_BT_AUTO || o.bt_mode > CUEV_BT_REORDERED)
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_solve: bad bt_mode %d", o.bt_mode);
+    if (o.dc_mode < CUEV_DC_AUTO || o.dc_mode > CUEV_DC_DEVICE)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_solve: bad dc_mode %d", o.dc_mode);
+    if (o.dc_leaf < 0 || o.dc_leaf == 1 || o.dc_leaf > 4096)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_solve: dc_leaf must be 0 or in [2, 4096] (got %d)",
+                  o.dc_leaf);
     int b = o.bandwidth, k = o.block;
     CUEV_TRY(resolve_blocking(n, b, k));
     if (k > n) k = (int)round_up(n, b); // one block covers the matrix
@@ -62,7 +67,7 @@ int solve_entry(T *A, int n, T *eval, T *evec, hipStream_t s, cuev_solve_timer *
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_solve: bad precision %d", o.precision);
     Context<T> *ctx = nullptr;
     CUEV_TRY(context_acquire<T>(ctx, n, b, k, s));
-    const int st = solve_native<T>(*ctx, A, eval, evec, timer, o.bt_mode);
+    const int st = solve_native<T>(*ctx, A, eval, evec, timer, o.bt_mode, o.dc_mode, o.dc_leaf);
     if (st == CUEV_STATUS_SUCCESS) {
         context_release<T>(ctx);
     } else {
@@ -261,6 +266,25 @@ int cuev_handle_destroy(cuev_handle_t h) {
     context_destroy<float>(h->f);
     context_destroy<double>(h->d);
     delete h;
+    return CUEV_STATUS_SUCCESS;
+}
+
+int cuev_handle_set_dc(cuev_handle_t h, int dc_mode, int dc_leaf) {
+    detail::clear_error();
+    if (!h) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "handle_set_dc: null handle");
+    if (dc_mode != CUEV_DC_HOST && dc_mode != CUEV_DC_DEVICE)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "handle_set_dc: dc_mode must be host or device (got %d)",
+                  dc_mode);
+    if (dc_leaf < 0 || dc_leaf == 1 || dc_leaf > 4096)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "handle_set_dc: dc_leaf must be 0 or in [2, 4096]");
+    if (h->f) {
+        h->f->dc_mode = dc_mode;
+        h->f->dc_leaf = dc_leaf;
+    }
+    if (h->d) {
+        h->d->dc_mode = dc_mode;
+        h->d->dc_leaf = dc_leaf;
+    }
     return CUEV_STATUS_SUCCESS;
 }
 

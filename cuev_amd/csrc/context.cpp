@@ -53,8 +53,36 @@ template <typename T> int context_create(Context<T> *&out, int n, int b, int k, 
     return CUEV_STATUS_SUCCESS;
 }
 
+template <typename T> int dc_ensure(Context<T> &ctx, bool need_q) {
+    const int n = ctx.ws.n;
+    if (!ctx.dc) CUEV_TRY(dc_state_create<T>(ctx.dc, n));
+    if (!ctx.dc_stream) CUEV_HIP(hipStreamCreateWithFlags(&ctx.dc_stream, hipStreamNonBlocking));
+    if (!ctx.dc_pool || (need_q && !ctx.dcQ)) {
+        if (ctx.dc_pool) {
+            (void)hipFree(ctx.dc_pool);
+            ctx.dc_pool = nullptr;
+            ctx.dcQ = ctx.dcQn = nullptr;
+        }
+        const size_t sq = sizeof(T) * (size_t)n * n;
+        const size_t bytes = (need_q ? 2 : 1) * ((sq + 255) & ~size_t(255));
+        if (hipMalloc(&ctx.dc_pool, bytes) != hipSuccess) {
+            ctx.dc_pool = nullptr;
+            (void)hipGetLastError();
+            CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "dc: hipMalloc(%zu) for the D&C buffers failed", bytes);
+        }
+        char *p = static_cast<char *>(ctx.dc_pool);
+        ctx.dcQn = reinterpret_cast<T *>(p);
+        if (need_q) ctx.dcQ = reinterpret_cast<T *>(p + ((sq + 255) & ~size_t(255)));
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
 template <typename T> void context_destroy(Context<T> *ctx) {
     if (!ctx) return;
+    if (ctx->dc_stream) (void)hipStreamSynchronize(ctx->dc_stream);
+    dc_state_destroy(ctx->dc);
+    if (ctx->dc_pool) (void)hipFree(ctx->dc_pool);
+    if (ctx->dc_stream) (void)hipStreamDestroy(ctx->dc_stream);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->h_d) (void)hipHostFree(ctx->h_d);
@@ -123,5 +151,7 @@ template int context_acquire<float>(Context<float> *&, int, int, int, hipStream_
 template int context_acquire<double>(Context<double> *&, int, int, int, hipStream_t);
 template void context_release<float>(Context<float> *);
 template void context_release<double>(Context<double> *);
+template int dc_ensure<float>(Context<float> &, bool);
+template int dc_ensure<double>(Context<double> &, bool);
 
 } // namespace cuev

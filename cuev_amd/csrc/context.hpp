@@ -1,6 +1,7 @@
 // context.hpp — a workspace plus the host staging of the tridiagonal stage.
 #pragma once
 
+#include "dc.hpp"
 #include "workspace.hpp"
 
 namespace cuev {
@@ -18,7 +19,19 @@ template <typename T> struct Context {
     i64 liwork = 0;
     hipEvent_t ev[12] = {};
     int dtype = 0;
+    // device-resident D&C, allocated on first use: state, Q and Qn (n x n each),
+    // and a stream of its own so it can run beside the back-transform
+    DcState<T> *dc = nullptr;
+    T *dcQ = nullptr;
+    T *dcQn = nullptr;
+    void *dc_pool = nullptr;
+    hipStream_t dc_stream = nullptr;
+    int dc_mode = 1; // stage API default: host ?stedc
+    int dc_leaf = 0;
 };
+
+// Allocate the device D&C state and buffers (need_q: also the separate Q).
+template <typename T> int dc_ensure(Context<T> &ctx, bool need_q);
 
 template <typename T> int context_create(Context<T> *&ctx, int n, int b, int k, hipStream_t s);
 template <typename T> void context_destroy(Context<T> *ctx);

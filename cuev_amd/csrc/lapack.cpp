@@ -47,6 +47,8 @@ bool try_lib(const std::string &path, const char *prefix) {
     g_fns.sstedc = reinterpret_cast<LapackFns::sstedc_t>(s);
     g_fns.dsyevd = reinterpret_cast<LapackFns::dsyevd_t>(sym("dsyevd_"));
     g_fns.dsygvd = reinterpret_cast<LapackFns::dsygvd_t>(sym("dsygvd_"));
+    g_fns.dsteqr = reinterpret_cast<LapackFns::dsteqr_t>(sym("dsteqr_"));
+    g_fns.ssteqr = reinterpret_cast<LapackFns::ssteqr_t>(sym("ssteqr_"));
     g_fns.name = path;
     g_fns.handle = h;
     return true;

@@ -24,6 +24,12 @@ struct LapackFns {
     typedef void (*dsygvd_t)(const int *, const char *, const char *, const int *, double *,
                              const int *, double *, const int *, double *, double *, const int *,
                              int *, const int *, int *);
+    typedef void (*dsteqr_t)(const char *, const int *, double *, double *, double *, const int *,
+                             double *, int *);
+    typedef void (*ssteqr_t)(const char *, const int *, float *, float *, float *, const int *,
+                             float *, int *);
+    dsteqr_t dsteqr = nullptr;
+    ssteqr_t ssteqr = nullptr;
     dstedc_t dstedc = nullptr;
     sstedc_t sstedc = nullptr;
     dsyevd_t dsyevd = nullptr;

@@ -19,6 +19,7 @@
 #include "backtransform.hpp"
 #include "bulge.hpp"
 #include "dbbr.hpp"
+#include "dc.hpp"
 #include "gemm.hpp"
 #include "lapack.hpp"
 #include "profile.hpp"
@@ -32,6 +33,8 @@ void options_defaults(cuev_options &o) {
     o.precision = CUEV_PRECISION_NATIVE;
     o.refine_sweeps = 0;
     o.bt_mode = CUEV_BT_AUTO;
+    o.dc_mode = CUEV_DC_AUTO;
+    o.dc_leaf = 0;
     o.refine_tol = 0.0;
 }
 
@@ -59,6 +62,14 @@ int tridi_dc(Context<T> &ctx, T *d, T *e, T *eval, T *evec, i64 ldv, double *dc_
         if (!std::isfinite((double)ctx.h_d[i]) || (i < n - 1 && !std::isfinite((double)ctx.h_e[i])))
             CUEV_FAIL(CUEV_STATUS_NOT_FINITE, "tridiagonal matrix has a non-finite entry at %d", i);
     auto t0 = clk::now();
+    if (ctx.dc_mode == CUEV_DC_DEVICE) {
+        CUEV_TRY(dc_ensure(ctx, true));
+        Scratch gs{ws.gscratch, ws.gscratch_bytes};
+        CUEV_TRY(dc_solve<T>(*ctx.dc, s, n, ctx.h_d, ctx.h_e, evec, ldv, eval, ctx.dcQ, ws.M, ctx.dcQn,
+                             gs, ctx.dc_leaf));
+        if (dc_ms) *dc_ms = ms_since(t0, clk::now());
+        return CUEV_STATUS_SUCCESS;
+    }
     CUEV_TRY(stedc<T>(n, ctx.h_d, ctx.h_e, ctx.h_z, n, ctx.h_work, ctx.lwork, ctx.h_iwork,
                       ctx.liwork));
     if (dc_ms) *dc_ms = ms_since(t0, clk::now());
@@ -87,7 +98,8 @@ int back_transform(Context<T> &ctx, const T *Y, const T *W, const T *U, T *evec,
 }
 
 template <typename T>
-int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode) {
+int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode,
+                 int dc_mode, int dc_leaf) {
     Workspace<T> &ws = ctx.ws;
     const int n = ws.n;
     hipStream_t s = ws.stream;
@@ -95,6 +107,8 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
     const auto wall0 = clk::now();
     if (bt_mode == CUEV_BT_AUTO) bt_mode = (n >= 2048) ? CUEV_BT_REORDERED : CUEV_BT_DIRECT;
     const bool reordered = bt_mode == CUEV_BT_REORDERED;
+    if (dc_mode == CUEV_DC_AUTO) dc_mode = (n >= 1024) ? CUEV_DC_DEVICE : CUEV_DC_HOST;
+    const bool dev_dc = dc_mode == CUEV_DC_DEVICE;
 
     CUEV_HIP(hipEventRecord(ev[0], s));
     CUEV_TRY(dbbr_reduce<T>(ws, A, ws.B));
@@ -120,24 +134,41 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
         if (!std::isfinite((double)ctx.h_d[i]) || (i < n - 1 && !std::isfinite((double)ctx.h_e[i])))
             CUEV_FAIL(CUEV_STATUS_NOT_FINITE,
                       "tridiagonal matrix has a non-finite entry at %d (input not finite?)", i);
+    Scratch gs{ws.gscratch, ws.gscratch_bytes};
+    if (dev_dc) CUEV_TRY(dc_ensure(ctx, reordered));
     const auto dc0 = clk::now();
-    CUEV_TRY(stedc<T>(n, ctx.h_d, ctx.h_e, ctx.h_z, n, ctx.h_work, ctx.lwork, ctx.h_iwork,
-                      ctx.liwork));
+    if (dev_dc) {
+        // A (consumed by the reduction) and evec (written last) serve as the D&C's
+        // work buffers. Reordered: the D&C runs on its own stream beside the
+        // back-transform still queued on s, without split-K (its scratch is s's).
+        if (reordered)
+            CUEV_TRY(dc_solve<T>(*ctx.dc, ctx.dc_stream, n, ctx.h_d, ctx.h_e, ctx.dcQ, n, eval, A, evec,
+                                 ctx.dcQn, Scratch{}, dc_leaf));
+        else
+            CUEV_TRY(dc_solve<T>(*ctx.dc, s, n, ctx.h_d, ctx.h_e, evec, n, eval, A, ws.M, ctx.dcQn, gs,
+                                 dc_leaf));
+    } else {
+        CUEV_TRY(stedc<T>(n, ctx.h_d, ctx.h_e, ctx.h_z, n, ctx.h_work, ctx.lwork, ctx.h_iwork,
+                          ctx.liwork));
+    }
     const auto dc1 = clk::now();
     CUEV_HIP(hipEventRecord(ev[6], s));
-    CUEV_HIP(hipMemcpyAsync(eval, ctx.h_d, sizeof(T) * n, hipMemcpyHostToDevice, s));
-    Scratch gs{ws.gscratch, ws.gscratch_bytes};
+    if (!dev_dc) CUEV_HIP(hipMemcpyAsync(eval, ctx.h_d, sizeof(T) * n, hipMemcpyHostToDevice, s));
     if (reordered) {
-        // Q_d into W (free after sbr_back), then evec = X Q_d
-        CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
+        // Q_d (device D&C: already resident; host D&C: into W, free after
+        // sbr_back), then evec = X Q_d
+        const T *Qd = dev_dc ? ctx.dcQ : ws.W;
+        if (!dev_dc)
+            CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipEventRecord(ev[7], s));
         {
             prof::Scope sc(s, "bt.gemm", 2.0 * (double)n * n * n);
-            CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), ws.M, ws.ld, ws.W, ws.ld, T(0), evec, n, gs));
+            CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gs));
         }
         CUEV_HIP(hipEventRecord(ev[8], s));
     } else {
-        CUEV_HIP(hipMemcpyAsync(evec, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
+        if (!dev_dc)
+            CUEV_HIP(hipMemcpyAsync(evec, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipEventRecord(ev[7], s));
         CUEV_TRY(bc_back<T>(ws, ws.U, evec, n));
         CUEV_HIP(hipEventRecord(ev[4], s));
@@ -178,8 +209,8 @@ template int back_transform<float>(Context<float> &, const float *, const float 
 template int back_transform<double>(Context<double> &, const double *, const double *,
                                     const double *, double *, cuev_solve_timer *);
 template int solve_native<float>(Context<float> &, float *, float *, float *, cuev_solve_timer *,
-                                 int);
+                                 int, int, int);
 template int solve_native<double>(Context<double> &, double *, double *, double *,
-                                  cuev_solve_timer *, int);
+                                  cuev_solve_timer *, int, int, int);
 
 } // namespace cuev

@@ -7,7 +7,8 @@ namespace cuev {
 
 void options_defaults(cuev_options &o);
 
-// Host D&C on device (d, e): eval <- eigenvalues, evec (ld ldv) <- Q_d.
+// D&C on device (d, e): eval <- eigenvalues, evec (ld ldv) <- Q_d; host ?stedc
+// or the device-resident form, as ctx.dc_mode says.
 template <typename T>
 int tridi_dc(Context<T> &ctx, T *d, T *e, T *eval, T *evec, i64 ldv, double *dc_ms);
 // evec <- Q_s Q_b evec (direct order).
@@ -16,6 +17,7 @@ int back_transform(Context<T> &ctx, const T *Y, const T *W, const T *U, T *evec,
                    cuev_solve_timer *timer);
 // Full solve in the precision T.
 template <typename T>
-int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode);
+int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode,
+                 int dc_mode, int dc_leaf);
 
 } // namespace cuev

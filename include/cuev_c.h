@@ -100,12 +100,22 @@ typedef enum cuev_bt_mode_t {
     CUEV_BT_REORDERED = 2  /* build Q_s Q_b on the GPU while the host D&C runs, then one GEMM */
 } cuev_bt_mode_t;
 
+typedef enum cuev_dc_mode_t {
+    CUEV_DC_AUTO = 0,   /* device for n >= 1024, host otherwise */
+    CUEV_DC_HOST = 1,   /* host LAPACK ?stedc, Q_d uploaded (the reference's north-star form) */
+    CUEV_DC_DEVICE = 2  /* divide and conquer with Q resident on the GPU: host deflation
+                           bookkeeping and leaves, device secular roots, vectors and GEMMs
+                           (the split of the reference's hybrid, src/cuda/tridi.cu:125-452) */
+} cuev_dc_mode_t;
+
 typedef struct cuev_options {
     int bandwidth;       /* b in {16, 32, 64}; 0 selects the default (64 for n >= 4096, else 32) */
     int block;           /* k, a multiple of b; 0 selects 512 (clamped to n) */
     int precision;       /* cuev_precision_t */
     int refine_sweeps;   /* mixed form: maximum sweeps; 0 selects 3 */
     int bt_mode;         /* cuev_bt_mode_t */
+    int dc_mode;         /* cuev_dc_mode_t */
+    int dc_leaf;         /* device D&C: largest leaf solved by host ?steqr; 0 selects 64 */
     double refine_tol;   /* mixed form: stop once the FP64 relative residual
                             ||AV - V diag(lambda)||_F / (n ||A||_F) is below this; 0 selects 2e-14 */
 } cuev_options;
@@ -185,8 +195,10 @@ int cuev_d_dbbr_reduce(cuev_handle_t h, double *A, double *B);
 /* Band -> tridiagonal. B is overwritten; reflectors go to the handle's U. */
 int cuev_s_bc_chase(cuev_handle_t h, float *B, float *d, float *e);
 int cuev_d_bc_chase(cuev_handle_t h, double *B, double *d, double *e);
-/* Host divide and conquer (LAPACK dstedc/sstedc) on device d, e; writes
-   eval (ascending) and the tridiagonal eigenvectors Q_d into evec (ld n). */
+/* Divide and conquer on device d, e; writes eval (ascending) and the
+   tridiagonal eigenvectors Q_d into evec (ld n). Host LAPACK ?stedc by default;
+   cuev_handle_set_dc selects the device-resident form. */
+int cuev_handle_set_dc(cuev_handle_t h, int dc_mode, int dc_leaf);
 int cuev_s_tridi_dc(cuev_handle_t h, float *d, float *e, float *eval, float *evec);
 int cuev_d_tridi_dc(cuev_handle_t h, double *d, double *e, double *eval, double *evec);
 /* evec <- Q_s Q_b evec (evec in: Q_d). */

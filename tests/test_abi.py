@@ -47,9 +47,13 @@ def test_status_strings(L):
 
 def test_options_defaults(L):
     import cuev_amd
-    o = cuev_amd.Options(7, 7, 7, 7, 7, 7.0)
+    o = cuev_amd.Options(7, 7, 7, 7, 7, 7, 7, 7.0)
     L.cuev_options_init(ctypes.byref(o))
-    assert (o.bandwidth, o.block, o.precision, o.refine_sweeps, o.bt_mode, o.refine_tol) == (0, 0, 0, 0, 0, 0.0)
+    assert (o.bandwidth, o.block, o.precision, o.refine_sweeps, o.bt_mode, o.dc_mode, o.dc_leaf,
+            o.refine_tol) == (0, 0, 0, 0, 0, 0, 0, 0.0)
+    # 7 ints, then the double at offset 32
+    assert ctypes.sizeof(cuev_amd.Options) == 40
+    assert cuev_amd.Options.refine_tol.offset == 32
 
 
 def test_timer_layout_matches_header():
