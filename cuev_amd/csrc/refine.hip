@@ -37,6 +37,7 @@
 #include "bulge.hpp"
 #include "context.hpp"
 #include "dbbr.hpp"
+#include "dc.hpp"
 #include "gemm.hpp"
 #include "lapack.hpp"
 #include "profile.hpp"
@@ -177,10 +178,15 @@ struct MixedBuffers {
     int64_t *h_iwork = nullptr;
     i64 lwork = 0, liwork = 0;
     void *pool = nullptr;
+    DcState<double> *dc = nullptr; // device D&C (FP64), created on first use
+    hipStream_t dc_stream = nullptr;
 };
 
 void mixed_free(MixedBuffers *m) {
     if (!m) return;
+    if (m->dc_stream) (void)hipStreamSynchronize(m->dc_stream);
+    dc_state_destroy(m->dc);
+    if (m->dc_stream) (void)hipStreamDestroy(m->dc_stream);
     if (m->pool) (void)hipFree(m->pool);
     if (m->h_d) (void)hipHostFree(m->h_d);
     if (m->h_e) (void)hipHostFree(m->h_e);
@@ -371,13 +377,28 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         if (!std::isfinite(mb.h_d[i]) || !std::isfinite(mb.h_e[i]))
             CUEV_FAIL(CUEV_STATUS_NOT_FINITE, "mixed: non-finite tridiagonal entry at %d", i);
     }
+    const int dc_mode = opt.dc_mode == CUEV_DC_AUTO ? (n >= 1024 ? CUEV_DC_DEVICE : CUEV_DC_HOST)
+                                                   : opt.dc_mode;
+    if (dc_mode == CUEV_DC_DEVICE) {
+        if (!mb.dc) CUEV_TRY(dc_state_create<double>(mb.dc, n));
+        if (!mb.dc_stream) CUEV_HIP(hipStreamCreateWithFlags(&mb.dc_stream, hipStreamNonBlocking));
+    }
     const auto dc0 = clk::now();
-    CUEV_TRY(stedc<double>(n, mb.h_d, mb.h_e, mb.h_z, n, mb.h_work, mb.lwork, mb.h_iwork, mb.liwork));
+    if (dc_mode == CUEV_DC_DEVICE) {
+        // Q_d into S and the eigenvalues into lam, on a stream of its own beside the
+        // FP32 back-transform; T1, G and evec (written last) are its work buffers
+        CUEV_TRY(dc_solve<double>(*mb.dc, mb.dc_stream, n, mb.h_d, mb.h_e, mb.S, n, mb.lam, mb.T1, mb.G,
+                                  evec, Scratch{}, opt.dc_leaf));
+    } else {
+        CUEV_TRY(stedc<double>(n, mb.h_d, mb.h_e, mb.h_z, n, mb.h_work, mb.lwork, mb.h_iwork, mb.liwork));
+    }
     const auto dc1 = clk::now();
     CUEV_HIP(hipEventRecord(ev[5], s));
     // X = (double)(X32 * (float)Q_d): Q_d through S (FP64) into W (FP32, free after sbr_back)
-    CUEV_HIP(hipMemcpyAsync(mb.S, mb.h_z, 8 * (size_t)n * n, hipMemcpyHostToDevice, s));
-    CUEV_HIP(hipMemcpyAsync(mb.lam, mb.h_d, 8 * (size_t)n, hipMemcpyHostToDevice, s));
+    if (dc_mode != CUEV_DC_DEVICE) {
+        CUEV_HIP(hipMemcpyAsync(mb.S, mb.h_z, 8 * (size_t)n * n, hipMemcpyHostToDevice, s));
+        CUEV_HIP(hipMemcpyAsync(mb.lam, mb.h_d, 8 * (size_t)n, hipMemcpyHostToDevice, s));
+    }
     CUEV_TRY((convert2d<double, float>(s, mb.S, n, ws.W, ws.ld, n, n)));
     {
         prof::Scope sc(s, "mixed.bt_gemm32", 2.0 * (double)n * n * n);

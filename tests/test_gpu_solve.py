@@ -123,7 +123,12 @@ def test_invalid_arguments(torch_dev):
     assert L.cuev_dsymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(o)) == 1
     o.bandwidth, o.block = 32, 48
     assert L.cuev_dsymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(o)) == 1
-    assert L.cuev_ssymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(cv.Options(0, 0, 1, 0, 0, 0.0))) == 8
+    assert L.cuev_ssymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(cv.Options(precision=1))) == 8
+    for field, bad in (("dc_mode", 3), ("dc_mode", -1), ("dc_leaf", 1), ("dc_leaf", 5000)):
+        o = cv.Options()
+        L.cuev_options_init(ctypes.byref(o))
+        setattr(o, field, bad)
+        assert L.cuev_dsymm_eig_solve_ex(p, 4, p, p, s, None, ctypes.byref(o)) == 1, (field, bad)
 
 
 def test_non_finite_input(torch_dev):
