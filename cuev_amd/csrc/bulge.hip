@@ -40,7 +40,7 @@
 namespace cuev {
 namespace {
 
-constexpr int kBCThreads = 256;
+constexpr int kBCThreads = 512; // 8 waves: two per SIMD, so LDS latency overlaps
 constexpr int kDone = 0x3fffffff; // prog value of a finished sweep
 
 template <typename T, int B>
@@ -50,18 +50,24 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
     constexpr int b = B;
     constexpr int NEW_PER_THREAD = 2 * B * B / kBCThreads; // b new columns of 2b rows per hop
     const int ldb = 2 * b;       // packed band leading dimension
-    const int WLD = 2 * b + 2;   // LDS leading dimension of a window column (padded)
+    // LDS leading dimension of a window column: odd, so that reading along a row
+    // (stride WLD doubles across lanes) spreads over the banks
+    const int WLD = 2 * b + 1;
     const int NSLOT = 2 * b;     // ring of window columns: matrix column j lives in slot j % NSLOT
+    constexpr int RPL = (3 * B + 63) / 64; // rows of p per lane
+    constexpr int PLD = 64 * RPL;          // stride of the per-wave p partials
+    constexpr int NW = kBCThreads / 64;    // waves per workgroup
     T *win = reinterpret_cast<T *>(smem_raw);          // [NSLOT][WLD]
     T *sw = win + (size_t)NSLOT * WLD;                 // [b]   reflector w
-    T *sp = sw + b;                                    // [3b]  p, then y
+    T *sp = sw + b;                                    // [3b]  p
+    T *spart = sp + 3 * b;                             // [NW][PLD] partial sums of p
+    __shared__ T sred[NW];
     __shared__ double s_beta;
-    __shared__ double s_kappa;
     __shared__ int s_go;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // window element A(i, j) (i >= j, both in the current window)
-    auto W_ = [&](int i, int j) -> T & { return win[(j % NSLOT) * WLD + (i - j)]; };
+    auto W_ = [&](int i, int j) -> T & { return win[(j & (NSLOT - 1)) * WLD + (i - j)]; };
     // copy packed columns [c0, c1] between global memory and their ring slots
     auto load_cols = [&](int c0, int c1) {
         if (c1 < c0) return;
@@ -69,7 +75,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         const T *src = Bp + (i64)c0 * ldb;
         for (int idx = tid; idx < cnt; idx += kBCThreads) {
             const int jj = idx / ldb, d = idx - jj * ldb;
-            win[((c0 + jj) % NSLOT) * WLD + d] = src[idx];
+            win[((c0 + jj) & (NSLOT - 1)) * WLD + d] = src[idx];
         }
     };
     // exactly b new columns: all loads issued before any LDS store (one round trip)
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         for (int u = 0; u < NEW_PER_THREAD; ++u) {
             const int idx = tid + u * kBCThreads;
             const int jj = idx / ldb, d = idx - jj * ldb;
-            win[((c0 + jj) % NSLOT) * WLD + d] = v[u];
+            win[((c0 + jj) & (NSLOT - 1)) * WLD + d] = v[u];
         }
     };
     auto store_cols = [&](int c0, int c1) {
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         T *dst = Bp + (i64)c0 * ldb;
         for (int idx = tid; idx < cnt; idx += kBCThreads) {
             const int jj = idx / ldb, d = idx - jj * ldb;
-            dst[idx] = win[((c0 + jj) % NSLOT) * WLD + d];
+            dst[idx] = win[((c0 + jj) & (NSLOT - 1)) * WLD + d];
         }
     };
 
@@ -158,50 +164,82 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             }
             __syncthreads();
 
-            // ---- p_i = 2 sum_k A(i, r0+k) w_k for i in [cq, hi]
-            const int nrow = hi - cq + 1;
-            for (int t = tid; t < nrow; t += kBCThreads) {
-                const int i = cq + t;
-                T acc = T(0);
-                for (int q = 0; q < L; ++q) {
-                    const int k = r0 + q;
-                    const T a = (i >= k) ? W_(i, k) : W_(k, i);
-                    acc += a * sw[q];
-                }
-                sp[t] = T(2) * acc;
-            }
-            __syncthreads();
-
-            // ---- kappa = w^T p ; y = p - kappa w
-            if (wave == 0) {
-                const double v = (lane < L) ? (double)sw[lane] * (double)sp[(r0 - cq) + lane] : 0.0;
-                const double kap = wave_sum(v);
-                if (lane == 0) s_kappa = kap;
-            }
-            __syncthreads();
+            // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow: each wave sums an eighth
+            // of the k range for all rows (RPL rows per lane), partials combined below
+            const int nrow = hi - cq + 1; // <= 3b
             {
-                const T kap = (T)s_kappa;
-                for (int t = tid; t < L; t += kBCThreads) sp[(r0 - cq) + t] -= kap * sw[t];
+                const int kper = (L + NW - 1) / NW;
+                const int k0 = wave * kper, k1 = min(L, k0 + kper);
+                T acc[RPL];
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) acc[r] = T(0);
+#pragma unroll 4
+                for (int q = k0; q < k1; ++q) {
+                    const int k = r0 + q;
+                    const T wk = sw[q];
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const int i = cq + lane + 64 * r;
+                        if (lane + 64 * r < nrow) {
+                            const T a = (i >= k) ? W_(i, k) : W_(k, i);
+                            acc[r] += a * wk;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < RPL; ++r)
+                    if (lane + 64 * r < nrow) spart[wave * PLD + lane + 64 * r] = acc[r];
+            }
+            __syncthreads();
+            // ---- p, and kappa = w^T p over the reflector's rows [o, o + L)
+            const int o = r0 - cq;
+            {
+                T kv = T(0);
+                if (tid < nrow) {
+                    T ps = T(0);
+#pragma unroll
+                    for (int v = 0; v < NW; ++v) ps += spart[v * PLD + tid];
+                    const T pv = T(2) * ps;
+                    sp[tid] = pv;
+                    if (tid >= o && tid < o + L) kv = pv * sw[tid - o];
+                }
+                kv = wave_sum(kv);
+                if (lane == 0) sred[wave] = kv;
             }
             __syncthreads();
 
-            // ---- rank-2 update A(i,j) -= w_i y_j + y_i w_j on the window (i >= j, i <= hi)
-            const int ncol = r1 - cq + 1;
-            for (int idx = tid; idx < ncol * ldb; idx += kBCThreads) {
-                const int jj = idx / ldb, d = idx - jj * ldb;
-                const int j = cq + jj, i = j + d;
-                if (i > hi) continue;
-                const bool wi_nz = (i >= r0 && i <= r1), wj_nz = (j >= r0);
-                if (!wi_nz && !wj_nz) continue;
-                T *a = &win[(j % NSLOT) * WLD + d];
-                if (jj == 0) {
-                    // eliminated column: exact result H x = beta e_0
-                    *a = (i == r0) ? (T)s_beta : T(0);
-                    continue;
+            // ---- rank-2 update A -= w y^T + y w^T with y = p - kappa w, on the three
+            // regions of the window where w_i or w_j is non-zero:
+            //   R1: rows r0+a (a < L)      x columns cq+c, 0 <= c < o  (w_j = 0); c = 0 is the
+            //       eliminated column, set to its exact value beta e_0
+            //   R2: rows r0+a (a < L)      x columns r0+c, c <= a     (both non-zero)
+            //   R3: rows r1+1+a (a < m3)   x columns r0+c, c < L      (w_i = 0)
+            // thread = (row a, column group cg); the column value is a broadcast read
+            {
+                T kap = T(0);
+#pragma unroll
+                for (int v = 0; v < NW; ++v) kap += sred[v];
+                constexpr int NG = kBCThreads / B;
+                const int a = tid % B, cg = tid / B;
+                const int m3 = hi - r1;
+                const T wa = (a < L) ? sw[a] : T(0);
+                const T ya = (a < L) ? sp[o + a] - kap * wa : T(0);
+                const T y3 = (a < m3) ? sp[o + L + a] : T(0);
+                if (a < L) {
+                    for (int cc = cg; cc < o; cc += NG) {
+                        T &e = W_(r0 + a, cq + cc);
+                        if (cc == 0) e = (a == 0) ? (T)s_beta : T(0);
+                        else e -= wa * sp[cc];
+                    }
+                    for (int cc = cg; cc <= a; cc += NG) {
+                        const T wc = sw[cc];
+                        const T yc = sp[o + cc] - kap * wc;
+                        W_(r0 + a, r0 + cc) -= wa * yc + ya * wc;
+                    }
                 }
-                const T wi = wi_nz ? sw[i - r0] : T(0);
-                const T wj = wj_nz ? sw[j - r0] : T(0);
-                *a -= wi * sp[j - cq] + sp[i - cq] * wj;
+                if (a < m3) {
+                    for (int cc = cg; cc < L; cc += NG) W_(r1 + 1 + a, r0 + cc) -= y3 * sw[cc];
+                }
             }
             __syncthreads();
 
@@ -235,7 +273,8 @@ __global__ void bc_extract_kernel(const T *Bp, T *d, T *e, int n, int b) {
 }
 
 template <typename T> size_t bc_lds_bytes(int b) {
-    return sizeof(T) * ((size_t)2 * b * (2 * b + 2) + b + 3 * b);
+    const size_t pld = 64 * (size_t)((3 * b + 63) / 64);
+    return sizeof(T) * ((size_t)2 * b * (2 * b + 1) + b + 3 * b + (kBCThreads / 64) * pld);
 }
 
 } // namespace

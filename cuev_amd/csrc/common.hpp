@@ -97,13 +97,22 @@ __device__ __forceinline__ unsigned counter_load_relaxed(const unsigned *p) {
 }
 
 // Workgroup-wide publish/consume of global data across workgroups.
-// publish: every wave writes back its stores (release fence at agent scope:
-// buffer_wbl2 + waitcnt, no invalidate), then the workgroup barrier, after which
-// one thread may store the flag. consume: after one thread has observed the
-// flag with a relaxed poll, it issues the acquire fence (invalidates the CU's
-// L1 and the non-coherent L2 lines) before the barrier releases the others.
+// publish: every wave waits until its own global stores have been acknowledged
+// by the L2 (s_waitcnt vmcnt(0); a plain workgroup barrier waits only for LDS),
+// then the workgroup barrier; after it ONE thread performs the agent-scope
+// release (the flag or counter atomic with release order: one L2 write-back,
+// buffer_wbl2, that covers every wave's stores, then the flag). consume: after
+// one thread has observed the flag with a relaxed poll, it issues the acquire
+// fence (invalidates the CU's L1 and the non-coherent L2 lines) before the
+// barrier releases the others.
+__device__ __forceinline__ void wave_stores_done() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST); // no compiler motion of memory ops across
+    // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): gfx9 encoding, vmcnt in [3:0] and [15:14]
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 __device__ __forceinline__ void wg_publish_fence() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wave_stores_done();
     __syncthreads();
 }
 

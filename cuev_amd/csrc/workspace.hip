@@ -1,6 +1,8 @@
 // workspace.hip — workspace pool allocation.
 #include "workspace.hpp"
 
+#include <vector>
+
 #include "gemm.hpp"
 #include "panel_qr.hpp"
 #include "util.hpp"
@@ -69,6 +71,21 @@ template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k,
     off += al((size_t)n * sizeof(int));
     const size_t oerr = off;
     off += al(64);
+    // bc_back block reflectors: group g covers sweeps [16g, 16g+16) (sweeps 0..n-3), and
+    // has as many hops as its first sweep, (n - 2 - 16g) / b + 1
+    std::vector<i64> bt_off;
+    if (n >= 3) {
+        ws.bt_groups = (int)ceil_div(n - 2, kBtGroup);
+        bt_off.resize(ws.bt_groups + 1);
+        bt_off[0] = 0;
+        for (int g = 0; g < ws.bt_groups; ++g)
+            bt_off[g + 1] = bt_off[g] + (n - 2 - g * kBtGroup) / b + 1;
+        ws.bt_blocks = bt_off[ws.bt_groups];
+    }
+    const size_t oBtT = off;
+    off += al((size_t)ws.bt_blocks * kBtGroup * kBtGroup * S);
+    const size_t oBtO = off;
+    off += al((bt_off.size() + 1) * sizeof(i64));
     ws.pool_bytes = off;
 
     hipError_t e = hipMalloc(&ws.pool, ws.pool_bytes);
@@ -94,6 +111,10 @@ template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k,
     ws.counter = reinterpret_cast<unsigned *>(base + oc);
     ws.prog = reinterpret_cast<int *>(base + op);
     ws.err = reinterpret_cast<int *>(base + oerr);
+    ws.BtT = reinterpret_cast<T *>(base + oBtT);
+    ws.bt_off = reinterpret_cast<i64 *>(base + oBtO);
+    if (!bt_off.empty())
+        CUEV_HIP(hipMemcpy(ws.bt_off, bt_off.data(), bt_off.size() * sizeof(i64), hipMemcpyHostToDevice));
     // U slack beyond ld*n is read (masked) by bc_back; give it defined contents.
     CUEV_TRY(fill2d<T>(s, ws.U, (i64)nu, (i64)nu, 1, T(0), T(0)));
     CUEV_TRY(fill_int(s, ws.err, 1, 0));

@@ -41,10 +41,19 @@ template <typename T> struct Workspace {
     unsigned *counter = nullptr; // panel-QR grid barrier counter
     int *prog = nullptr;         // bc_chase per-sweep progress (n ints)
     int *err = nullptr;          // device error flag
+    // bc_back block reflectors: sweeps grouped kBtGroup at a time; block (g, h) holds
+    // the hop-h reflectors of sweeps [16g, 16g+16) as I - V T V^T. T factors are
+    // kBtGroup^2 each, block (g, h) at index bt_off[g] + h.
+    T *BtT = nullptr;
+    i64 *bt_off = nullptr; // bt_groups + 1 prefix offsets
+    int bt_groups = 0;
+    i64 bt_blocks = 0;
 
     void *pool = nullptr;
     size_t pool_bytes = 0;
 };
+
+constexpr int kBtGroup = 16; // sweeps per bc_back block reflector
 
 template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k, hipStream_t s);
 template <typename T> void workspace_free(Workspace<T> &ws);
