@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--b", type=int, default=64)
     ap.add_argument("--k", type=int, default=512)
     ap.add_argument("--bt", default="auto", choices=["auto", "direct", "reordered"])
+    ap.add_argument("--dc", default="auto", choices=["auto", "host", "device"],
+                    help="tridiagonal D&C: host LAPACK dstedc or the device-resident form")
     ap.add_argument("--mixed", action="store_true", help="also time the mixed FP32+refinement form")
     ap.add_argument("--cpu-n", type=int, default=6144, help="n of the dsyevd CPU sample (0: skip)")
     ap.add_argument("--no-check", action="store_true")
@@ -101,7 +103,7 @@ def main():
         work.copy_(A0)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        lam, V, tm = cv.symm_eig_solve(work, overwrite=True, bandwidth=b, block=k, bt_mode=a.bt,
+        lam, V, tm = cv.symm_eig_solve(work, overwrite=True, bandwidth=b, block=k, bt_mode=a.bt, dc_mode=a.dc,
                                        precision=precision, timer=True)
         torch.cuda.synchronize()
         return time.perf_counter() - t, lam, V, tm
@@ -224,7 +226,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{a.workload}: n={n}, b={b}, k={k}, A=Q diag(lam) Q^T" if a.workload == "q16k"
                    else f"{a.workload}: n={n}, b={b}, k={k}", "n": n, "b": b, "k": k,
-                   "bt_mode": a.bt, "parallelism": "single GPU"},
+                   "bt_mode": a.bt, "dc_mode": a.dc, "parallelism": "single GPU"},
         "stages_ms": {kk: round(v, 2) for kk, v in stages.items()},
         "effective_tflops_16_3": round(flops16_3 / mean_s / 1e12, 3),
         "effective_tflops_4_3": round(flops16_3 / 4.0 / mean_s / 1e12, 3),

@@ -25,14 +25,17 @@
 // kept in registers are at once the B operand of X and the accumulator of the
 // update.
 //
-// sbr_back (M <- Q_s M): DBBR panels last-first, M_b -= W_j (Y_j^T M_b), as the
-// reference (backtransform.cu:210-230), on the MFMA engine.
+// sbr_back (M <- Q_s M): the reference applies the DBBR panels last-first,
+// M_b -= W_j (Y_j^T M_b) (backtransform.cu:210-230), one rank-b update of M per
+// panel. Here the panels of one reduction block are folded into one block
+// reflector first, so M is updated once per block (rank k) on the MFMA engine.
 #include "backtransform.hpp"
 
 #include <algorithm>
 
 #include "gemm.hpp"
 #include "profile.hpp"
+#include "util.hpp"
 
 namespace cuev {
 namespace {
@@ -123,16 +126,77 @@ __global__ __launch_bounds__(kBTThreads) void bt_tfactor_kernel(const T *__restr
 template <typename T, int B> struct BtOperands {
     static constexpr int NKB = B / 16 + 1;
     T vx[4 * NKB]; // X = V^T M_J: A operand of chunk c, V[16(c/4) + kidx(c%4, q)][li]
-    T vu[NKB][4];  // M_J -= V Y: A operand of tile t, chunk r, -V[16t + li][kidx(r, q)]
+    T vu[NKB][4];  // M_J -= V Y: A operand of tile t, chunk r, V[16t + li][kidx(r, q)]
     T t[4];        // Y = T X: A operand of chunk r, T[li][kidx(r, q)]
 };
+
+// Position of a block in the application order (wave-uniform).
+struct BtPos {
+    int h0, g, h; // pass start hop, sweep group, hop
+};
+
+// Advance to the next block: hops of the group, then the group below, then the
+// next pass. Returns false after the last block.
+template <int B> __device__ __forceinline__ bool bt_next(BtPos &p, int n) {
+    constexpr int G = 128 / B;
+    if (p.h + 1 < p.h0 + G && p.g * kBtGroup + 1 + (p.h + 1) * B <= n - 1) {
+        ++p.h;
+        return true;
+    }
+    if (p.g > 0) {
+        --p.g;
+        p.h = p.h0;
+        return true;
+    }
+    p.h0 += G;
+    const int smax = min(n - 3, n - 2 - p.h0 * B);
+    if (smax < 0) return false;
+    p.g = smax / kBtGroup;
+    p.h = p.h0;
+    return true;
+}
+
+// Operands of block (g, h) in the MFMA lane layouts. Invalid entries are read
+// from zp (zeros), so every load is unconditional with no select after it and
+// the loads can stay in flight across the previous blocks' products.
+template <typename T, int B>
+__device__ __forceinline__ void bt_load_ops(BtOperands<T, B> &o, const T *__restrict__ U, i64 ldu,
+                                            const T *__restrict__ zp, const T *__restrict__ Tf,
+                                            const i64 *__restrict__ toff, int g, int h, int last,
+                                            int q, int li) {
+    using MB = MfmaBt<T>;
+    constexpr int NKB = B / 16 + 1;
+    const int s0 = g * kBtGroup;
+    const T *Ug = U + (i64)h * B;
+#pragma unroll
+    for (int c = 0; c < 4 * NKB; ++c) {
+        const int d = 16 * (c >> 2) + MB::kidx(c & 3, q) - li, sw = s0 + li;
+        const bool ok = d >= 0 && d < B && sw <= last;
+        o.vx[c] = *(ok ? Ug + d + (i64)sw * ldu : zp);
+    }
+#pragma unroll
+    for (int t = 0; t < NKB; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = MB::kidx(r, q);
+            const int d = 16 * t + li - j, sw = s0 + j;
+            const bool ok = d >= 0 && d < B && sw <= last;
+            o.vu[t][r] = *(ok ? Ug + d + (i64)sw * ldu : zp);
+        }
+    const T *Tb = Tf + (toff[g] + h) * (kBtGroup * kBtGroup);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o.t[r] = Tb[li + MB::kidx(r, q) * kBtGroup];
+}
 
 // M <- Q_b M for columns [0, ncols): one wave per 16 columns (see the header).
 // Each wave keeps the rows of the current pass's window, WS = G*b + 16 rows of
 // its 16 columns, in a private circular LDS buffer (row r in slot r mod WS):
 // the window slides up 16 rows per sweep group, so a group costs 16 rows in and
-// 16 rows out. The next block's V and T and the next group's entering rows are
-// loaded into registers while the current block computes.
+// 16 rows out. The operands of the next two blocks and the next group's
+// entering rows are loaded into registers while the current block computes.
+// Invalid operands and rows past n are read from zp: zeros past U's last column
+// (the workspace fills ldu*n + 2048 elements of U with zeros and every sweep
+// writes only its own column).
 template <typename T, int B>
 __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict__ U, i64 ldu,
                                                              const T *__restrict__ Tf,
@@ -147,116 +211,87 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
     constexpr int G = 128 / B;      // hops per pass
     constexpr int WS = G * B + 16;  // window rows
     constexpr int LDW = 17;         // LDS row stride: rows and 16-column reads both conflict-free
-    __shared__ T win_all[kBBThreads / 64][WS * LDW];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, li = lane & 15;
+    __shared__ T win[(kBBThreads / 64) * WS * LDW];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, q = lane >> 4, li = lane & 15;
     const int wcol = (blockIdx.x * (kBBThreads / 64) + wave) * 16;
     if (wcol >= ncols) return; // whole wave
-    T *win = win_all[wave];
-    const int last = n - 3;    // last sweep
-    for (int e = lane; e < WS * LDW; e += 64) win[e] = T(0);
+    const int wb = wave * (WS * LDW); // this wave's window
+    const int last = n - 3;           // last sweep
+    const T *zp = U + ldu * (i64)n;
+    for (int e = lane; e < WS * LDW; e += 64) win[wb + e] = T(0);
 
-    // 16 rows [r0, r0+16) of the wave's columns: lane = (row li, columns q + 4i)
-    auto load16 = [&](int r0, T (&v)[4]) {
-        const int r = r0 + li;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int col = wcol + q + 4 * i;
-            v[i] = (r < n && col < ncols) ? M[r + (i64)col * ldm] : T(0);
-        }
-    };
-    auto put16 = [&](int r0, const T (&v)[4]) {
-        const int s = (r0 + li) % WS;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) win[s * LDW + q + 4 * i] = v[i];
-    };
-    auto store16 = [&](int r0) {
-        const int r = r0 + li;
-        if (r >= n) return;
-        const int s = r % WS;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int col = wcol + q + 4 * i;
-            if (col < ncols) M[r + (i64)col * ldm] = win[s * LDW + q + 4 * i];
-        }
-    };
-    auto load_window = [&](int top) {
-        for (int c = 0; c < WS; c += 16) {
-            T v[4];
-            load16(top + c, v);
-            put16(top + c, v);
-        }
-    };
-    auto flush_window = [&](int top) {
-        for (int c = 0; c < WS; c += 16) store16(top + c);
-        wave_stores_done(); // the next pass reads these rows back through other lanes
-    };
-    auto load_ops = [&](int g, int h, Ops &o) {
-        const int s0 = g * NG;
-        const T *Ug = U + (i64)h * B;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const int d = 16 * (c >> 2) + MB::kidx(c & 3, q) - li, sw = s0 + li;
-            o.vx[c] = (d >= 0 && d < B && sw <= last) ? Ug[d + (i64)sw * ldu] : T(0);
-        }
-#pragma unroll
-        for (int t = 0; t < NKB; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int j = MB::kidx(r, q);
-                const int d = 16 * t + li - j, sw = s0 + j;
-                o.vu[t][r] = (d >= 0 && d < B && sw <= last) ? -Ug[d + (i64)sw * ldu] : T(0);
-            }
-        const T *Tb = Tf + (toff[g] + h) * (NG * NG);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) o.t[r] = Tb[li + MB::kidx(r, q) * NG];
-    };
-    // next block in the order: hops of the group, then the group below, then the next pass
-    auto next_block = [&](int &h0, int &g, int &h) -> bool {
-        if (h + 1 < h0 + G && g * NG + 1 + (h + 1) * B <= n - 1) {
-            ++h;
-            return true;
-        }
-        if (g > 0) {
-            --g;
-            h = h0;
-            return true;
-        }
-        h0 += G;
-        const int smax = min(last, n - 2 - h0 * B);
-        if (smax < 0) return false;
-        g = smax / NG;
-        h = h0;
-        return true;
-    };
+// rows [r0, r0+16) of the wave's 16 columns: lane = (row li, columns q + 4i)
+#define BT_LOAD16(r0, v)                                                                           \
+    do {                                                                                           \
+        const int r_ = (r0) + li;                                                                  \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                         \
+            const int col_ = wcol + q + 4 * i_;                                                    \
+            const bool ok_ = r_ >= 0 && r_ < n && col_ < ncols;                                    \
+            (v)[i_] = *(ok_ ? M + r_ + (i64)col_ * ldm : zp);                                      \
+        }                                                                                          \
+    } while (0)
+#define BT_PUT16(r0, v)                                                                            \
+    do {                                                                                           \
+        const int s_ = ((r0) + li) % WS;                                                           \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[wb + s_ * LDW + q + 4 * i_] = (v)[i_]; \
+    } while (0)
+#define BT_STORE16(r0)                                                                             \
+    do {                                                                                           \
+        const int r_ = (r0) + li;                                                                  \
+        if (r_ < n) {                                                                              \
+            const int s_ = r_ % WS;                                                                \
+            _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                     \
+                const int col_ = wcol + q + 4 * i_;                                                \
+                if (col_ < ncols) M[r_ + (i64)col_ * ldm] = win[wb + s_ * LDW + q + 4 * i_];      \
+            }                                                                                      \
+        }                                                                                          \
+    } while (0)
 
-    int h0 = 0, g = min(last, n - 2) / NG, h = 0;
-    int top = g * NG + 1 + h0 * B; // first window row
-    Ops cur, nxt;
-    load_ops(g, h, cur);
-    load_window(top);
-    T ent[4] = {T(0), T(0), T(0), T(0)};
+    BtPos p{0, min(last, n - 2) / NG, 0};
+    int top = p.g * NG + 1; // first window row
+    BtPos pn = p;
+    bool more = bt_next<B>(pn, n);
+    Ops cur, nxt, nx2;
+    bt_load_ops<T, B>(cur, U, ldu, zp, Tf, toff, p.g, p.h, last, q, li);
+    bt_load_ops<T, B>(nxt, U, ldu, zp, Tf, toff, more ? pn.g : p.g, more ? pn.h : p.h, last, q, li);
+    for (int c = 0; c < WS; c += 16) {
+        T v[4];
+        BT_LOAD16(top + c, v);
+        BT_PUT16(top + c, v);
+    }
+    T ent[4];
     for (;;) {
-        int h0n = h0, gn = g, hn = h;
-        const bool more = next_block(h0n, gn, hn);
-        if (more) load_ops(gn, hn, nxt);
-        if (h == h0 && g > 0) load16(top - 16, ent); // rows entering with the group below
+        BtPos pn2 = pn;
+        const bool more2 = more && bt_next<B>(pn2, n);
+        // unconditional (a repeat of this block when nothing follows): no branch
+        bt_load_ops<T, B>(nx2, U, ldu, zp, Tf, toff, more2 ? pn2.g : p.g, more2 ? pn2.h : p.h, last, q,
+                          li);
+        // rows entering with the group below (the same rows for every block of a group)
+        BT_LOAD16(top - 16, ent);
 
         // ---- block (g, h): rows [base, base + 16 NKB) of the window
-        const int base = g * NG + 1 + h * B;
+        const int base = p.g * NG + 1 + p.h * B;
         const int sb = base % WS;
         T m[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             int s = sb + 16 * (c >> 2) + MB::kidx(c & 3, q);
             if (s >= WS) s -= WS;
-            m[c] = win[s * LDW + li];
+            m[c] = win[wb + s * LDW + li];
         }
-        V4 x = {T(0), T(0), T(0), T(0)};
+        // X = V^T M_J in two independent chains (even and odd chunks)
+        V4 x = {T(0), T(0), T(0), T(0)}, x1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int c = 0; c < NC; ++c) x = MB::mma(cur.vx[c], m[c], x);
+        for (int c = 0; c < NC; c += 2) {
+            x = MB::mma(cur.vx[c], m[c], x);
+            x1 = MB::mma(cur.vx[c + 1], m[c + 1], x1);
+        }
+        x += x1;
         V4 y = {T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int r = 0; r < 4; ++r) y = MB::mma(cur.t[r], x[r], y);
+        y = -y; // M_J += V (-Y)
 #pragma unroll
         for (int t = 0; t < NKB; ++t) {
             V4 acc = {m[4 * t], m[4 * t + 1], m[4 * t + 2], m[4 * t + 3]};
@@ -266,31 +301,37 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
             for (int r = 0; r < 4; ++r) {
                 int s = sb + 16 * t + MB::kidx(r, q);
                 if (s >= WS) s -= WS;
-                win[s * LDW + li] = acc[r];
+                win[wb + s * LDW + li] = acc[r];
             }
         }
 
         // ---- window bookkeeping for the next block
-        if (!more) {
-            flush_window(top);
-            break;
-        }
-        if (h0n != h0) {
-            // pass done: write the window back, load the next pass's first window
-            flush_window(top);
-            top = gn * NG + 1 + h0n * B;
-            load_window(top);
-        } else if (gn != g) {
+        if (!more || pn.h0 != p.h0) {
+            // pass done: write the window back
+            for (int c = 0; c < WS; c += 16) BT_STORE16(top + c);
+            if (!more) break;
+            wave_stores_done(); // the next pass reads these rows back through other lanes
+            top = pn.g * NG + 1 + pn.h0 * B;
+            for (int c = 0; c < WS; c += 16) {
+                T v[4];
+                BT_LOAD16(top + c, v);
+                BT_PUT16(top + c, v);
+            }
+        } else if (pn.g != p.g) {
             // group g -> g-1: 16 rows leave at the bottom, 16 enter at the top
-            store16(top + WS - 16);
-            put16(top - 16, ent);
+            BT_STORE16(top + WS - 16);
+            BT_PUT16(top - 16, ent);
             top -= 16;
         }
-        h0 = h0n;
-        g = gn;
-        h = hn;
+        p = pn;
+        pn = pn2;
+        more = more2;
         cur = nxt;
+        nxt = nx2;
     }
+#undef BT_LOAD16
+#undef BT_PUT16
+#undef BT_STORE16
 }
 
 template <typename T, int B> int launch_bc_back(Workspace<T> &ws, const T *U, T *M, i64 ldm) {
@@ -326,21 +367,51 @@ template <typename T> int bc_back(Workspace<T> &ws, const T *U, T *M, i64 ldm) {
 }
 
 template <typename T> int sbr_back(Workspace<T> &ws, const T *Y, const T *W, T *M, i64 ldm) {
-    const int n = ws.n, b = ws.b;
+    const int n = ws.n, b = ws.b, k = ws.k;
     const i64 ld = ws.ld;
     if (ldm < n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "sbr_back: ldm < n");
     Scratch gs{ws.gscratch, ws.gscratch_bytes};
-    int jmax = -1;
-    for (int j = 0; j + b < n; j += b) jmax = j;
-    for (int j = jmax; j >= 0; j -= b) {
-        const int rows = n - (j + b);
-        const T *Yp = Y + (i64)j * ld + (j + b);
-        const T *Wp = W + (i64)j * ld + (j + b);
-        T *Mb = M + (j + b);
-        // Zt = Y_j^T M_b (b x n) ; M_b -= W_j Zt
-        prof::Scope sc(ws.stream, "bt.sbr_back", 4.0 * rows * (double)b * n);
-        CUEV_TRY(gemm<T>(ws.stream, 1, 0, b, n, rows, T(1), Yp, ld, Mb, ldm, T(0), ws.Z, b, gs));
-        CUEV_TRY(gemm<T>(ws.stream, 0, 0, rows, n, b, T(-1), Wp, ld, ws.Z, b, T(1), Mb, ldm, gs));
+    hipStream_t s = ws.stream;
+    // Blocks of the reduction (k columns, np = k/b panels), last block first. Block
+    // i applies H_i0 H_i1 ... = I - What Yhat^T with Yhat = [Y_i0 .. Y_inp-1] and
+    // What built left to right: What_t = W_t - What_{0:t} (Yhat_{0:t}^T W_t)
+    // (what panel t's W becomes once the earlier panels' reflectors are folded in).
+    // Then Zt = Yhat^T M (one b-row product per panel, over that panel's rows) and
+    // one rank-(np b) update M -= What Zt.
+    int ilast = -1;
+    for (int i = 0; i + b < n; i += k) ilast = i;
+    for (int i = ilast; i >= 0; i -= k) {
+        int np = 0;
+        while (np * b < k && i + np * b + b < n) ++np;
+        const int r0 = i + b; // first row of the block's reflectors
+        const i64 R0 = n - r0;
+        T *Wh = ws.Z + r0;    // What, rows r0.. (ld ld), column block t = panel t
+        {
+            prof::Scope sc(s, "bt.sbr_what");
+            for (int t = 0; t < np; ++t) {
+                const int j = i + t * b, rt = j + b; // panel t: columns j.., rows rt..
+                T *Wt = ws.Z + (i64)t * b * ld;
+                CUEV_TRY(fill2d<T>(s, Wt + r0, ld, rt - r0, b, T(0), T(0)));
+                CUEV_TRY(copy2d<T>(s, W + (i64)j * ld + rt, ld, Wt + rt, ld, n - rt, b));
+                if (t == 0) continue;
+                // C = Yhat_{0:t}^T W_t over rows rt.. (where W_t lives), t b x b
+                CUEV_TRY(gemm<T>(s, 1, 0, (i64)t * b, b, n - rt, T(1), Y + (i64)i * ld + rt, ld,
+                                 W + (i64)j * ld + rt, ld, T(0), ws.Dwk, (i64)t * b, gs));
+                // What_t -= What_{0:t} C over rows r0..
+                CUEV_TRY(gemm<T>(s, 0, 0, R0, b, (i64)t * b, T(-1), Wh, ld, ws.Dwk, (i64)t * b, T(1),
+                                 Wt + r0, ld, gs));
+            }
+        }
+        {
+            prof::Scope sc(s, "bt.sbr_back", 4.0 * (double)R0 * np * b * n);
+            for (int t = 0; t < np; ++t) {
+                const int j = i + t * b, rt = j + b;
+                CUEV_TRY(gemm<T>(s, 1, 0, b, n, n - rt, T(1), Y + (i64)j * ld + rt, ld, M + rt, ldm,
+                                 T(0), ws.Zt + (i64)t * b, k, gs));
+            }
+            CUEV_TRY(gemm<T>(s, 0, 0, R0, n, (i64)np * b, T(-1), Wh, ld, ws.Zt, k, T(1), M + r0, ldm,
+                             gs));
+        }
     }
     return CUEV_STATUS_SUCCESS;
 }
