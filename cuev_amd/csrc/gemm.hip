@@ -87,17 +87,112 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
     }
 }
 
-template <typename T, int BM, int BN>
-__global__ __launch_bounds__(kThreads) void gemm_tile_kernel(const KParams<T> p) {
+// Operand modes, fixed at compile time so that each kernel carries only the
+// loader it needs (register pressure decides how many waves share a SIMD).
+enum AMode { A_N = 0, A_T = 1, A_SYM = 2, A_CAT = 3 }; // A_CAT: two m-contiguous sources
+enum BMode { B_N = 0, B_T = 1, B_CATT = 2 };           // B_CATT: two n-contiguous sources
+
+// A tile (BM x BK at rows m0, k in [k0, k0+BK)) into registers, in the order
+// store_a expects. Pattern 0: m-contiguous (element (m,k) at p[m + k*ld]);
+// pattern 1: k-contiguous (at p[k + m*ld]); pattern 2: symmetric straddling the
+// diagonal (lower triangle stored).
+template <typename T, int BM, int NA>
+__device__ __forceinline__ void load_a(T (&ra)[NA], int pat, const T *pa, i64 lda, i64 m0, i64 k0,
+                                       i64 kend, i64 M, int tid) {
+    if (pat == 1) {
+        const int kk = tid % BK;
+        const i64 k = k0 + kk;
+        const bool kok = k < kend;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const i64 m = m0 + tid / BK + (kThreads / BK) * i;
+            ra[i] = (kok && m < M) ? pa[k + m * lda] : T(0);
+        }
+    } else {
+        const int ml = tid % BM;
+        const i64 m = m0 + ml;
+        const bool mok = m < M;
+        const int kk0 = tid / BM;
+        const T *base = pa + m + (k0 + kk0) * lda;
+        const i64 step = (i64)(kThreads / BM) * lda;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const i64 k = k0 + kk0 + (kThreads / BM) * i;
+            if (pat == 0) {
+                ra[i] = (mok && k < kend) ? base[i * step] : T(0);
+            } else {
+                const i64 idx = (m >= k) ? m + k * lda : k + m * lda;
+                ra[i] = (mok && k < kend) ? pa[idx] : T(0);
+            }
+        }
+    }
+}
+
+template <typename T, int BM, int NA>
+__device__ __forceinline__ void store_a(T (*sA)[BM + 1], const T (&ra)[NA], int pat, int tid) {
+    if (pat == 1) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) sA[tid % BK][tid / BK + (kThreads / BK) * i] = ra[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) sA[tid / BM + (kThreads / BM) * i][tid % BM] = ra[i];
+    }
+}
+
+// B tile (BK x BN at columns n0): pattern 0 k-contiguous (p[k + n*ld]), 1 n-contiguous.
+template <typename T, int BN, int NB>
+__device__ __forceinline__ void load_b(T (&rb)[NB], int pat, const T *pb, i64 ldb, i64 n0, i64 k0,
+                                       i64 kend, i64 N, int tid) {
+    if (pat == 0) {
+        const int kk = tid % BK;
+        const i64 k = k0 + kk;
+        const bool kok = k < kend;
+        const T *base = pb + k + (n0 + tid / BK) * ldb;
+        const i64 step = (i64)(kThreads / BK) * ldb;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const i64 n = n0 + tid / BK + (kThreads / BK) * i;
+            rb[i] = (kok && n < N) ? base[i * step] : T(0);
+        }
+    } else {
+        const int nl = tid % BN;
+        const i64 n = n0 + nl;
+        const bool nok = n < N;
+        const int kk0 = tid / BN;
+        const T *base = pb + n + (k0 + kk0) * ldb;
+        const i64 step = (i64)(kThreads / BN) * ldb;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const i64 k = k0 + kk0 + (kThreads / BN) * i;
+            rb[i] = (nok && k < kend) ? base[i * step] : T(0);
+        }
+    }
+}
+
+template <typename T, int BN, int NB>
+__device__ __forceinline__ void store_b(T (*sB)[BN + 1], const T (&rb)[NB], int pat, int tid) {
+    if (pat == 0) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) sB[tid % BK][tid / BK + (kThreads / BK) * i] = rb[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) sB[tid / BN + (kThreads / BN) * i][tid % BN] = rb[i];
+    }
+}
+
+// Block tile BM x BN, K step BK, 4 waves (2 x 2) each owning (BM/2) x (BN/2).
+// __launch_bounds__(256, 2): at most 256 VGPRs, so two waves share each SIMD and
+// one hides the other's LDS and barrier latency.
+template <typename T, int BM, int BN, int AMODE, int BMODE>
+__global__ __launch_bounds__(kThreads, 2) void gemm_tile_kernel(const KParams<T> p) {
     constexpr int TM = BM / 2, TN = BN / 2;
     constexpr int FM = TM / 16, FN = TN / 16;
     constexpr int NA = BM * BK / kThreads; // A elements per thread per k-tile
     constexpr int NB = BN * BK / kThreads;
-    constexpr int PAD = 1;
     using V4 = typename Mfma<T>::v4;
 
-    __shared__ T sA[2][BK][BM + PAD];
-    __shared__ T sB[2][BK][BN + PAD];
+    __shared__ T sA[2][BK][BM + 1];
+    __shared__ T sB[2][BK][BN + 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -105,8 +200,8 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(const KParams<T> p)
     const int wm = wave & 1, wn = wave >> 1;
 
     // XCD-contiguous remap of the block index
-    const int nb = gridDim.x, bid = blockIdx.x;
-    const int xcd = bid & 7, q = nb >> 3, rmd = nb & 7;
+    const int nbk = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, q = nbk >> 3, rmd = nbk & 7;
     const int t = xcd * q + min(xcd, rmd) + (bid >> 3);
     int tm, tn;
     tile_coords(t, p.tiles_m, p.tiles_n, p.lower, tm, tn);
@@ -123,99 +218,47 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(const KParams<T> p)
         for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
 
     T ra[NA], rb[NB];
-    int patA = 0; // 0: m-contiguous, 1: k-contiguous, 2: m-contiguous with symmetric select
-
+    int patA = 0, patB = 0;
     auto load_tiles = [&](i64 k0) {
-        // ---- A tile: rows [m0, m0+BM), k in [k0, k0+BK)
-        const T *pa;
-        i64 lda, ko;
-        if (p.asym) {
-            pa = p.a0;
-            lda = p.lda0;
-            ko = 0;
-            if (m0 >= k0 + BK - 1) patA = 0;
-            else if (m0 + BM - 1 <= k0) patA = 1;
-            else patA = 2;
+        // A operand source and pattern for this k-tile
+        const T *pa = p.a0;
+        i64 lda = p.lda0, ko = 0;
+        if constexpr (AMODE == A_N) {
+            patA = 0;
+        } else if constexpr (AMODE == A_T) {
+            patA = 1;
+        } else if constexpr (AMODE == A_SYM) {
+            patA = (m0 >= k0 + BK - 1) ? 0 : (m0 + BM - 1 <= k0) ? 1 : 2;
         } else {
-            const bool second = p.asplit >= 0 && k0 >= p.asplit;
-            pa = second ? p.a1 : p.a0;
-            lda = second ? p.lda1 : p.lda0;
-            ko = second ? p.asplit : 0;
-            patA = p.atrans ? 1 : 0;
-        }
-#pragma unroll
-        for (int i = 0; i < NA; ++i) {
-            int ml, kk;
-            if (patA == 1) {
-                ml = tid / BK + (kThreads / BK) * i;
-                kk = tid % BK;
-            } else {
-                ml = tid % BM;
-                kk = tid / BM + (kThreads / BM) * i;
-            }
-            const i64 m = m0 + ml, k = k0 + kk;
-            T v = T(0);
-            if (m < p.M && k < kend) {
-                if (patA == 0) v = pa[m + (k - ko) * lda];
-                else if (patA == 1) v = pa[(k - ko) + m * lda];
-                else v = (m >= k) ? pa[m + k * lda] : pa[k + m * lda];
-            }
-            ra[i] = v;
-        }
-        // ---- B tile: k in [k0, k0+BK), cols [n0, n0+BN)
-        {
-            const bool second = p.bsplit >= 0 && k0 >= p.bsplit;
-            const T *pb = second ? p.b1 : p.b0;
-            const i64 ldb = second ? p.ldb1 : p.ldb0;
-            const i64 ko2 = second ? p.bsplit : 0;
-#pragma unroll
-            for (int i = 0; i < NB; ++i) {
-                int nl, kk;
-                if (p.btrans) {
-                    nl = tid % BN;
-                    kk = tid / BN + (kThreads / BN) * i;
-                } else {
-                    nl = tid / BK + (kThreads / BK) * i;
-                    kk = tid % BK;
-                }
-                const i64 n = n0 + nl, k = k0 + kk;
-                T v = T(0);
-                if (n < p.N && k < kend)
-                    v = p.btrans ? pb[n + (k - ko2) * ldb] : pb[(k - ko2) + n * ldb];
-                rb[i] = v;
+            patA = 0;
+            if (k0 >= p.asplit) {
+                pa = p.a1;
+                lda = p.lda1;
+                ko = p.asplit;
             }
         }
-    };
-    auto store_tiles = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < NA; ++i) {
-            int ml, kk;
-            if (patA == 1) {
-                ml = tid / BK + (kThreads / BK) * i;
-                kk = tid % BK;
-            } else {
-                ml = tid % BM;
-                kk = tid / BM + (kThreads / BM) * i;
+        load_a<T, BM, NA>(ra, patA, pa, lda, m0, k0 - ko, kend - ko, p.M, tid);
+        const T *pb = p.b0;
+        i64 ldb = p.ldb0, kob = 0;
+        if constexpr (BMODE == B_N) {
+            patB = 0;
+        } else if constexpr (BMODE == B_T) {
+            patB = 1;
+        } else {
+            patB = 1;
+            if (k0 >= p.bsplit) {
+                pb = p.b1;
+                ldb = p.ldb1;
+                kob = p.bsplit;
             }
-            sA[buf][kk][ml] = ra[i];
         }
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            int nl, kk;
-            if (p.btrans) {
-                nl = tid % BN;
-                kk = tid / BN + (kThreads / BN) * i;
-            } else {
-                nl = tid / BK + (kThreads / BK) * i;
-                kk = tid % BK;
-            }
-            sB[buf][kk][nl] = rb[i];
-        }
+        load_b<T, BN, NB>(rb, patB, pb, ldb, n0, k0 - kob, kend - kob, p.N, tid);
     };
 
     if (nkt > 0) {
         load_tiles(kbeg);
-        store_tiles(0);
+        store_a<T, BM, NA>(sA[0], ra, patA, tid);
+        store_b<T, BN, NB>(sB[0], rb, patB, tid);
     }
     __syncthreads();
 
@@ -236,7 +279,10 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(const KParams<T> p)
 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(bf[j], af[i], acc[i][j]);
         }
-        if (more) store_tiles(cur ^ 1);
+        if (more) {
+            store_a<T, BM, NA>(sA[cur ^ 1], ra, patA, tid);
+            store_b<T, BN, NB>(sB[cur ^ 1], rb, patB, tid);
+        }
         __syncthreads();
     }
 
@@ -257,8 +303,8 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(const KParams<T> p)
                 if (partial) {
                     part[m + n * p.M] = v;
                 } else {
-                    T *c = p.C + m + n * p.ldc;
-                    *c = (p.beta == T(0)) ? p.alpha * v : p.alpha * v + p.beta * (*c);
+                    T *cp = p.C + m + n * p.ldc;
+                    *cp = (p.beta == T(0)) ? p.alpha * v : p.alpha * v + p.beta * (*cp);
                 }
             }
         }
@@ -280,12 +326,29 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int s
     *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
 }
 
-template <typename T, int BM, int BN>
-int launch_cfg(hipStream_t s, KParams<T> &p, int splits, int ntiles) {
+template <typename T, int BM, int BN, int AM, int BMD>
+int launch_modes(hipStream_t s, KParams<T> &p, int splits, int ntiles) {
     dim3 grid((unsigned)ntiles, (unsigned)splits);
-    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN>), grid, dim3(kThreads), 0, s, p);
+    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, AM, BMD>), grid, dim3(kThreads), 0, s, p);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T, int BM, int BN>
+int launch_cfg(hipStream_t s, KParams<T> &p, int splits, int ntiles, int am, int bm) {
+#define CUEV_GEMM_B(AMV)                                                                           \
+    switch (bm) {                                                                                  \
+    case B_N: return launch_modes<T, BM, BN, AMV, B_N>(s, p, splits, ntiles);                      \
+    case B_T: return launch_modes<T, BM, BN, AMV, B_T>(s, p, splits, ntiles);                      \
+    default: return launch_modes<T, BM, BN, AMV, B_CATT>(s, p, splits, ntiles);                    \
+    }
+    switch (am) {
+    case A_N: CUEV_GEMM_B(A_N)
+    case A_T: CUEV_GEMM_B(A_T)
+    case A_SYM: CUEV_GEMM_B(A_SYM)
+    default: CUEV_GEMM_B(A_CAT)
+    }
+#undef CUEV_GEMM_B
 }
 
 } // namespace
@@ -374,11 +437,20 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     }
 
     if (splits >= 1) {
+        int am, bm;
+        if (A.sym) am = A_SYM;
+        else if (A.split >= 0) am = A_CAT;
+        else am = A.trans ? A_T : A_N;
+        if (B.split >= 0) bm = B_CATT;
+        else bm = B.trans ? B_T : B_N;
+        if ((A.sym && (A.split >= 0 || A.trans)) || (A.split >= 0 && A.trans) ||
+            (B.split >= 0 && !B.trans) || B.sym)
+            CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "gemm: operand mode combination not implemented");
         int st;
-        if (BMs == 128 && BNs == 128) st = launch_cfg<T, 128, 128>(s, p, splits, (int)ntiles);
-        else if (BMs == 128) st = launch_cfg<T, 128, 64>(s, p, splits, (int)ntiles);
-        else if (BNs == 128) st = launch_cfg<T, 64, 128>(s, p, splits, (int)ntiles);
-        else st = launch_cfg<T, 64, 64>(s, p, splits, (int)ntiles);
+        if (BMs == 128 && BNs == 128) st = launch_cfg<T, 128, 128>(s, p, splits, (int)ntiles, am, bm);
+        else if (BMs == 128) st = launch_cfg<T, 128, 64>(s, p, splits, (int)ntiles, am, bm);
+        else if (BNs == 128) st = launch_cfg<T, 64, 128>(s, p, splits, (int)ntiles, am, bm);
+        else st = launch_cfg<T, 64, 64>(s, p, splits, (int)ntiles, am, bm);
         if (st) return st;
     }
     if (p.part != nullptr || splits == 0) {
