@@ -156,10 +156,30 @@ template <int B> __device__ __forceinline__ bool bt_next(BtPos &p, int n) {
     return true;
 }
 
+// Step back to the previous block of that order (the order of Q_b^T M, whose blocks
+// are the same reflectors in reverse, each transposed). Returns false before the first.
+template <int B> __device__ __forceinline__ bool bt_prev(BtPos &p, int n) {
+    constexpr int G = 128 / B;
+    if (p.h > p.h0) {
+        --p.h;
+        return true;
+    }
+    if (p.g < min(n - 3, n - 2 - p.h0 * B) / kBtGroup) {
+        ++p.g;
+        p.h = min(p.h0 + G - 1, (n - 2 - p.g * kBtGroup) / B);
+        return true;
+    }
+    if (p.h0 == 0) return false;
+    p.h0 -= G;
+    p.g = 0;
+    p.h = min(p.h0 + G - 1, (n - 2) / B);
+    return true;
+}
+
 // Operands of block (g, h) in the MFMA lane layouts. Invalid entries are read
 // from zp (zeros), so every load is unconditional with no select after it and
 // the loads can stay in flight across the previous blocks' products.
-template <typename T, int B>
+template <typename T, int B, bool TR>
 __device__ __forceinline__ void bt_load_ops(BtOperands<T, B> &o, const T *__restrict__ U, i64 ldu,
                                             const T *__restrict__ zp, const T *__restrict__ Tf,
                                             const i64 *__restrict__ toff, int g, int h, int last,
@@ -185,11 +205,12 @@ __device__ __forceinline__ void bt_load_ops(BtOperands<T, B> &o, const T *__rest
         }
     const T *Tb = Tf + (toff[g] + h) * (kBtGroup * kBtGroup);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o.t[r] = Tb[li + MB::kidx(r, q) * kBtGroup];
+    for (int r = 0; r < 4; ++r)
+        o.t[r] = TR ? Tb[MB::kidx(r, q) + li * kBtGroup] : Tb[li + MB::kidx(r, q) * kBtGroup];
 }
 
-// M <- Q_b M for columns [0, ncols): one wave per 16 columns (see the header).
-// Each wave keeps the rows of the current pass's window, WS = G*b + 16 rows of
+// M <- Q_b M (TR: M <- Q_b^T M, the blocks in reverse order with T^T) for
+// columns [0, ncols): one wave per 16 columns (see the header). Each wave keeps the rows of the current pass's window, WS = G*b + 16 rows of
 // its 16 columns, in a private circular LDS buffer (row r in slot r mod WS):
 // the window slides up 16 rows per sweep group, so a group costs 16 rows in and
 // 16 rows out. The operands of the next two blocks and the next group's
@@ -197,7 +218,7 @@ __device__ __forceinline__ void bt_load_ops(BtOperands<T, B> &o, const T *__rest
 // Invalid operands and rows past n are read from zp: zeros past U's last column
 // (the workspace fills ldu*n + 2048 elements of U with zeros and every sweep
 // writes only its own column).
-template <typename T, int B>
+template <typename T, int B, bool TR>
 __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict__ U, i64 ldu,
                                                              const T *__restrict__ Tf,
                                                              const i64 *__restrict__ toff, T *M,
@@ -248,13 +269,22 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
         }                                                                                          \
     } while (0)
 
-    BtPos p{0, min(last, n - 2) / NG, 0};
-    int top = p.g * NG + 1; // first window row
+    BtPos p;
+    if (TR) {
+        p.h0 = ((n - 2) / B) / G * G; // last pass
+        p.g = 0;
+        p.h = min(p.h0 + G - 1, (n - 2) / B);
+    } else {
+        p.h0 = 0;
+        p.g = min(last, n - 2) / NG;
+        p.h = 0;
+    }
+    int top = p.g * NG + 1 + p.h0 * B; // first window row
     BtPos pn = p;
-    bool more = bt_next<B>(pn, n);
+    bool more = TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n);
     Ops cur, nxt, nx2;
-    bt_load_ops<T, B>(cur, U, ldu, zp, Tf, toff, p.g, p.h, last, q, li);
-    bt_load_ops<T, B>(nxt, U, ldu, zp, Tf, toff, more ? pn.g : p.g, more ? pn.h : p.h, last, q, li);
+    bt_load_ops<T, B, TR>(cur, U, ldu, zp, Tf, toff, p.g, p.h, last, q, li);
+    bt_load_ops<T, B, TR>(nxt, U, ldu, zp, Tf, toff, more ? pn.g : p.g, more ? pn.h : p.h, last, q, li);
     for (int c = 0; c < WS; c += 16) {
         T v[4];
         BT_LOAD16(top + c, v);
@@ -263,12 +293,13 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
     T ent[4];
     for (;;) {
         BtPos pn2 = pn;
-        const bool more2 = more && bt_next<B>(pn2, n);
+        const bool more2 = more && (TR ? bt_prev<B>(pn2, n) : bt_next<B>(pn2, n));
         // unconditional (a repeat of this block when nothing follows): no branch
-        bt_load_ops<T, B>(nx2, U, ldu, zp, Tf, toff, more2 ? pn2.g : p.g, more2 ? pn2.h : p.h, last, q,
-                          li);
-        // rows entering with the group below (the same rows for every block of a group)
-        BT_LOAD16(top - 16, ent);
+        bt_load_ops<T, B, TR>(nx2, U, ldu, zp, Tf, toff, more2 ? pn2.g : p.g, more2 ? pn2.h : p.h, last,
+                              q, li);
+        // rows entering with the next group (the same rows for every block of a group):
+        // above the window going down the groups, below it going up (TR)
+        BT_LOAD16(TR ? top + WS : top - 16, ent);
 
         // ---- block (g, h): rows [base, base + 16 NKB) of the window
         const int base = p.g * NG + 1 + p.h * B;
@@ -318,10 +349,17 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
                 BT_PUT16(top + c, v);
             }
         } else if (pn.g != p.g) {
-            // group g -> g-1: 16 rows leave at the bottom, 16 enter at the top
-            BT_STORE16(top + WS - 16);
-            BT_PUT16(top - 16, ent);
-            top -= 16;
+            if (TR) {
+                // group g -> g+1: 16 rows leave at the top, 16 enter at the bottom
+                BT_STORE16(top);
+                BT_PUT16(top + WS, ent);
+                top += 16;
+            } else {
+                // group g -> g-1: 16 rows leave at the bottom, 16 enter at the top
+                BT_STORE16(top + WS - 16);
+                BT_PUT16(top - 16, ent);
+                top -= 16;
+            }
         }
         p = pn;
         pn = pn2;
@@ -334,36 +372,73 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
 #undef BT_STORE16
 }
 
-template <typename T, int B> int launch_bc_back(Workspace<T> &ws, const T *U, T *M, i64 ldm) {
+template <typename T, int B>
+int launch_bc_back(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, bool tr) {
     const int n = ws.n;
     if (ws.bt_blocks <= 0) return CUEV_STATUS_SUCCESS;
     {
-        prof::Scope sc(ws.stream, "bt.bc_back_t", (double)ws.bt_blocks * kBtGroup * kBtGroup * B);
+        prof::Scope sc(s, "bt.bc_back_t", (double)ws.bt_blocks * kBtGroup * kBtGroup * B);
         hipLaunchKernelGGL((bt_tfactor_kernel<T, B>), dim3((unsigned)ceil_div(ws.bt_blocks, 4)),
-                           dim3(kBTThreads), 0, ws.stream, U, ws.ldu, ws.bt_off, ws.bt_groups,
-                           ws.bt_blocks, n, ws.BtT);
+                           dim3(kBTThreads), 0, s, U, ws.ldu, ws.bt_off, ws.bt_groups, ws.bt_blocks, n,
+                           ws.BtT);
         CUEV_LAUNCH_CHECK();
     }
     // 4 flops per reflector entry per column: sum_s (n-1-s) entries ~ n^2/2
-    prof::Scope sc(ws.stream, "bt.bc_back", 2.0 * (double)n * n * n);
+    prof::Scope sc(s, "bt.bc_back", 2.0 * (double)n * n * n);
     const i64 blocks = ceil_div(n, 16 * (kBBThreads / 64));
-    hipLaunchKernelGGL((bc_back_kernel<T, B>), dim3((unsigned)blocks), dim3(kBBThreads), 0, ws.stream,
-                       U, ws.ldu, ws.BtT, ws.bt_off, M, ldm, n, n);
+    if (tr)
+        hipLaunchKernelGGL((bc_back_kernel<T, B, true>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s, U,
+                           ws.ldu, ws.BtT, ws.bt_off, M, ldm, n, n);
+    else
+        hipLaunchKernelGGL((bc_back_kernel<T, B, false>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
+                           U, ws.ldu, ws.BtT, ws.bt_off, M, ldm, n, n);
     CUEV_LAUNCH_CHECK();
+    return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T> int bc_back_any(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, bool tr) {
+    if (ldm < ws.n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_back: ldm < n");
+    if (ws.n < 3) return CUEV_STATUS_SUCCESS;
+    switch (ws.b) {
+    case 16: return launch_bc_back<T, 16>(ws, s, U, M, ldm, tr);
+    case 32: return launch_bc_back<T, 32>(ws, s, U, M, ldm, tr);
+    case 64: return launch_bc_back<T, 64>(ws, s, U, M, ldm, tr);
+    default: CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_back: unsupported b=%d", ws.b);
+    }
+}
+
+// What for the block of panels i .. i+np-1 into ws.Z (rows r0 = i+b .., ld ws.ld)
+template <typename T>
+int build_what(Workspace<T> &ws, hipStream_t s, const T *Y, const T *W, int i, int np, Scratch gs) {
+    const int n = ws.n, b = ws.b;
+    const i64 ld = ws.ld;
+    const int r0 = i + b;
+    const i64 R0 = n - r0;
+    T *Wh = ws.Z + r0;
+    for (int t = 0; t < np; ++t) {
+        const int j = i + t * b, rt = j + b; // panel t: columns j.., rows rt..
+        T *Wt = ws.Z + (i64)t * b * ld;
+        CUEV_TRY(fill2d<T>(s, Wt + r0, ld, rt - r0, b, T(0), T(0)));
+        CUEV_TRY(copy2d<T>(s, W + (i64)j * ld + rt, ld, Wt + rt, ld, n - rt, b));
+        if (t == 0) continue;
+        // C = Yhat_{0:t}^T W_t over rows rt.. (where W_t lives), t b x b
+        CUEV_TRY(gemm<T>(s, 1, 0, (i64)t * b, b, n - rt, T(1), Y + (i64)i * ld + rt, ld,
+                         W + (i64)j * ld + rt, ld, T(0), ws.Dwk, (i64)t * b, gs));
+        // What_t -= What_{0:t} C over rows r0..
+        CUEV_TRY(gemm<T>(s, 0, 0, R0, b, (i64)t * b, T(-1), Wh, ld, ws.Dwk, (i64)t * b, T(1), Wt + r0,
+                         ld, gs));
+    }
     return CUEV_STATUS_SUCCESS;
 }
 
 } // namespace
 
 template <typename T> int bc_back(Workspace<T> &ws, const T *U, T *M, i64 ldm) {
-    if (ldm < ws.n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_back: ldm < n");
-    if (ws.n < 3) return CUEV_STATUS_SUCCESS;
-    switch (ws.b) {
-    case 16: return launch_bc_back<T, 16>(ws, U, M, ldm);
-    case 32: return launch_bc_back<T, 32>(ws, U, M, ldm);
-    case 64: return launch_bc_back<T, 64>(ws, U, M, ldm);
-    default: CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_back: unsupported b=%d", ws.b);
-    }
+    return bc_back_any<T>(ws, ws.stream, U, M, ldm, false);
+}
+
+template <typename T> int bc_back_t(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm) {
+    return bc_back_any<T>(ws, s, U, M, ldm, true);
 }
 
 template <typename T> int sbr_back(Workspace<T> &ws, const T *Y, const T *W, T *M, i64 ldm) {
@@ -385,39 +460,61 @@ template <typename T> int sbr_back(Workspace<T> &ws, const T *Y, const T *W, T *
         while (np * b < k && i + np * b + b < n) ++np;
         const int r0 = i + b; // first row of the block's reflectors
         const i64 R0 = n - r0;
-        T *Wh = ws.Z + r0;    // What, rows r0.. (ld ld), column block t = panel t
         {
             prof::Scope sc(s, "bt.sbr_what");
-            for (int t = 0; t < np; ++t) {
-                const int j = i + t * b, rt = j + b; // panel t: columns j.., rows rt..
-                T *Wt = ws.Z + (i64)t * b * ld;
-                CUEV_TRY(fill2d<T>(s, Wt + r0, ld, rt - r0, b, T(0), T(0)));
-                CUEV_TRY(copy2d<T>(s, W + (i64)j * ld + rt, ld, Wt + rt, ld, n - rt, b));
-                if (t == 0) continue;
-                // C = Yhat_{0:t}^T W_t over rows rt.. (where W_t lives), t b x b
-                CUEV_TRY(gemm<T>(s, 1, 0, (i64)t * b, b, n - rt, T(1), Y + (i64)i * ld + rt, ld,
-                                 W + (i64)j * ld + rt, ld, T(0), ws.Dwk, (i64)t * b, gs));
-                // What_t -= What_{0:t} C over rows r0..
-                CUEV_TRY(gemm<T>(s, 0, 0, R0, b, (i64)t * b, T(-1), Wh, ld, ws.Dwk, (i64)t * b, T(1),
-                                 Wt + r0, ld, gs));
-            }
+            CUEV_TRY(build_what<T>(ws, s, Y, W, i, np, gs));
         }
+        prof::Scope sc(s, "bt.sbr_back", 4.0 * (double)R0 * np * b * n);
+        for (int t = 0; t < np; ++t) {
+            const int j = i + t * b, rt = j + b;
+            CUEV_TRY(gemm<T>(s, 1, 0, b, n, n - rt, T(1), Y + (i64)j * ld + rt, ld, M + rt, ldm, T(0),
+                             ws.Zt + (i64)t * b, k, gs));
+        }
+        CUEV_TRY(gemm<T>(s, 0, 0, R0, n, (i64)np * b, T(-1), ws.Z + r0, ld, ws.Zt, k, T(1), M + r0, ldm,
+                         gs));
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
+template <typename T>
+int sbr_back_t(Workspace<T> &ws, hipStream_t s, const T *Y, const T *W, T *M, i64 ldm) {
+    const int n = ws.n, b = ws.b, k = ws.k;
+    const i64 ld = ws.ld;
+    if (ldm < n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "sbr_back_t: ldm < n");
+    Scratch gs{ws.gscratch, ws.gscratch_bytes};
+    // Q_s^T = ... (I - Yhat_1 What_1^T)(I - Yhat_0 What_0^T): blocks first to last,
+    // M -= Yhat (What^T M) with Yhat the block's Y columns, zero above each panel's
+    // rows (copied into ws.Yh, since Y there holds nothing defined)
+    for (int i = 0; i + b < n; i += k) {
+        int np = 0;
+        while (np * b < k && i + np * b + b < n) ++np;
+        const int r0 = i + b;
+        const i64 R0 = n - r0;
         {
-            prof::Scope sc(s, "bt.sbr_back", 4.0 * (double)R0 * np * b * n);
+            prof::Scope sc(s, "bt.sbr_what");
+            CUEV_TRY(build_what<T>(ws, s, Y, W, i, np, gs));
             for (int t = 0; t < np; ++t) {
                 const int j = i + t * b, rt = j + b;
-                CUEV_TRY(gemm<T>(s, 1, 0, b, n, n - rt, T(1), Y + (i64)j * ld + rt, ld, M + rt, ldm,
-                                 T(0), ws.Zt + (i64)t * b, k, gs));
+                T *Yt = ws.Yh + (i64)t * b * ld;
+                CUEV_TRY(fill2d<T>(s, Yt + r0, ld, rt - r0, b, T(0), T(0)));
+                CUEV_TRY(copy2d<T>(s, Y + (i64)j * ld + rt, ld, Yt + rt, ld, n - rt, b));
             }
-            CUEV_TRY(gemm<T>(s, 0, 0, R0, n, (i64)np * b, T(-1), Wh, ld, ws.Zt, k, T(1), M + r0, ldm,
-                             gs));
         }
+        prof::Scope sc(s, "bt.sbr_back", 4.0 * (double)R0 * np * b * n);
+        CUEV_TRY(gemm<T>(s, 1, 0, (i64)np * b, n, R0, T(1), ws.Z + r0, ld, M + r0, ldm, T(0), ws.Zt, k, gs));
+        CUEV_TRY(gemm<T>(s, 0, 0, R0, n, (i64)np * b, T(-1), ws.Yh + r0, ld, ws.Zt, k, T(1), M + r0, ldm,
+                         gs));
     }
     return CUEV_STATUS_SUCCESS;
 }
 
 template int bc_back<float>(Workspace<float> &, const float *, float *, i64);
 template int bc_back<double>(Workspace<double> &, const double *, double *, i64);
+template int bc_back_t<float>(Workspace<float> &, hipStream_t, const float *, float *, i64);
+template int bc_back_t<double>(Workspace<double> &, hipStream_t, const double *, double *, i64);
+template int sbr_back_t<float>(Workspace<float> &, hipStream_t, const float *, const float *, float *, i64);
+template int sbr_back_t<double>(Workspace<double> &, hipStream_t, const double *, const double *, double *,
+                                i64);
 template int sbr_back<float>(Workspace<float> &, const float *, const float *, float *, i64);
 template int sbr_back<double>(Workspace<double> &, const double *, const double *, double *, i64);
 

@@ -77,8 +77,19 @@ template <typename T> int dc_ensure(Context<T> &ctx, bool need_q) {
     return CUEV_STATUS_SUCCESS;
 }
 
+template <typename T> int bt_ensure(Context<T> &ctx) {
+    if (!ctx.bt_stream) CUEV_HIP(hipStreamCreateWithFlags(&ctx.bt_stream, hipStreamNonBlocking));
+    for (auto &e : ctx.xev)
+        if (!e) CUEV_HIP(hipEventCreate(&e));
+    return CUEV_STATUS_SUCCESS;
+}
+
 template <typename T> void context_destroy(Context<T> *ctx) {
     if (!ctx) return;
+    if (ctx->bt_stream) (void)hipStreamSynchronize(ctx->bt_stream);
+    for (auto &e : ctx->xev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->bt_stream) (void)hipStreamDestroy(ctx->bt_stream);
     if (ctx->dc_stream) (void)hipStreamSynchronize(ctx->dc_stream);
     dc_state_destroy(ctx->dc);
     if (ctx->dc_pool) (void)hipFree(ctx->dc_pool);
@@ -152,6 +163,8 @@ template int context_acquire<double>(Context<double> *&, int, int, int, hipStrea
 template void context_release<float>(Context<float> *);
 template void context_release<double>(Context<double> *);
 template int dc_ensure<float>(Context<float> &, bool);
+template int bt_ensure<float>(Context<float> &);
+template int bt_ensure<double>(Context<double> &);
 template int dc_ensure<double>(Context<double> &, bool);
 
 } // namespace cuev

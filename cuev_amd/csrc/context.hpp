@@ -26,12 +26,18 @@ template <typename T> struct Context {
     T *dcQn = nullptr;
     void *dc_pool = nullptr;
     hipStream_t dc_stream = nullptr;
+    // reordered back-transform: X^T = Q_b^T Q_s^T built on a stream of its own,
+    // Q_s^T while the bulge chasing runs (events: start/end of each half)
+    hipStream_t bt_stream = nullptr;
+    hipEvent_t xev[4] = {};
     int dc_mode = 1; // stage API default: host ?stedc
     int dc_leaf = 0;
 };
 
 // Allocate the device D&C state and buffers (need_q: also the separate Q).
 template <typename T> int dc_ensure(Context<T> &ctx, bool need_q);
+// Create the back-transform stream and its events.
+template <typename T> int bt_ensure(Context<T> &ctx);
 
 template <typename T> int context_create(Context<T> *&ctx, int n, int b, int k, hipStream_t s);
 template <typename T> void context_destroy(Context<T> *ctx);

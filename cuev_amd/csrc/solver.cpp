@@ -110,6 +110,8 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
     if (dc_mode == CUEV_DC_AUTO) dc_mode = (n >= 1024) ? CUEV_DC_DEVICE : CUEV_DC_HOST;
     const bool dev_dc = dc_mode == CUEV_DC_DEVICE;
 
+    if (reordered) CUEV_TRY(bt_ensure(ctx));
+    hipStream_t bs = ctx.bt_stream;
     CUEV_HIP(hipEventRecord(ev[0], s));
     CUEV_TRY(dbbr_reduce<T>(ws, A, ws.B));
     CUEV_HIP(hipEventRecord(ev[1], s));
@@ -120,12 +122,18 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
     CUEV_HIP(hipMemcpyAsync(ctx.h_e, ws.e, sizeof(T) * n, hipMemcpyDeviceToHost, s));
     CUEV_HIP(hipEventRecord(ev[3], s));
     if (reordered) {
-        // X = Q_s Q_b built on the GPU while the host runs the D&C
-        CUEV_TRY(fill2d<T>(s, ws.M, ws.ld, n, n, T(0), T(1)));
-        CUEV_TRY(bc_back<T>(ws, ws.U, ws.M, ws.ld));
-        CUEV_HIP(hipEventRecord(ev[4], s));
-        CUEV_TRY(sbr_back<T>(ws, ws.Y, ws.W, ws.M, ws.ld));
-        CUEV_HIP(hipEventRecord(ev[5], s));
+        // X^T = Q_b^T Q_s^T in M on the back-transform stream: Q_s^T needs only the
+        // band reduction and runs beside the bulge chasing (which occupies half the
+        // CUs, LDS-bound); Q_b^T follows the chasing and runs beside the D&C.
+        CUEV_HIP(hipStreamWaitEvent(bs, ev[1], 0));
+        CUEV_HIP(hipEventRecord(ctx.xev[0], bs));
+        CUEV_TRY(fill2d<T>(bs, ws.M, ws.ld, n, n, T(0), T(1)));
+        CUEV_TRY(sbr_back_t<T>(ws, bs, ws.Y, ws.W, ws.M, ws.ld));
+        CUEV_HIP(hipEventRecord(ctx.xev[1], bs));
+        CUEV_HIP(hipStreamWaitEvent(bs, ev[2], 0));
+        CUEV_HIP(hipEventRecord(ctx.xev[2], bs));
+        CUEV_TRY(bc_back_t<T>(ws, bs, ws.U, ws.M, ws.ld));
+        CUEV_HIP(hipEventRecord(ctx.xev[3], bs));
     }
     // wait for (d, e) only; the reordered back-transform keeps running on the GPU.
     // The device error flag is checked once the stream has drained.
@@ -155,15 +163,16 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
     CUEV_HIP(hipEventRecord(ev[6], s));
     if (!dev_dc) CUEV_HIP(hipMemcpyAsync(eval, ctx.h_d, sizeof(T) * n, hipMemcpyHostToDevice, s));
     if (reordered) {
-        // Q_d (device D&C: already resident; host D&C: into W, free after
-        // sbr_back), then evec = X Q_d
+        // Q_d (device D&C: already resident; host D&C: into W, free once the
+        // back-transform stream is done), then evec = X Q_d = (X^T)^T Q_d
+        CUEV_HIP(hipStreamWaitEvent(s, ctx.xev[3], 0));
         const T *Qd = dev_dc ? ctx.dcQ : ws.W;
         if (!dev_dc)
             CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipEventRecord(ev[7], s));
         {
             prof::Scope sc(s, "bt.gemm", 2.0 * (double)n * n * n);
-            CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gs));
+            CUEV_TRY(gemm<T>(s, 1, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gs));
         }
         CUEV_HIP(hipEventRecord(ev[8], s));
     } else {
@@ -185,8 +194,8 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
         timer->dc_ms = (float)ms_since(dc0, dc1);
         timer->total_ms = (float)ms_since(wall0, wall1);
         if (reordered) {
-            timer->bt_qb_ms = ev_ms(ev[3], ev[4]);
-            timer->bt_qs_ms = ev_ms(ev[4], ev[5]);
+            timer->bt_qs_ms = ev_ms(ctx.xev[0], ctx.xev[1]);
+            timer->bt_qb_ms = ev_ms(ctx.xev[2], ctx.xev[3]);
             timer->overlap_ms = timer->bt_qb_ms + timer->bt_qs_ms;
             timer->bt_copyin_ms = ev_ms(ev[6], ev[7]);
             timer->bt_copyout_ms = ev_ms(ev[7], ev[8]);

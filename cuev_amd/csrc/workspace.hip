@@ -45,6 +45,8 @@ template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k,
     off += al((size_t)n * k * S);
     const size_t oZt = off;
     off += al((size_t)n * k * S);
+    const size_t oYh = off;
+    off += al((size_t)n * k * S);
     const size_t oD = off;
     off += al((size_t)2 * k * b * S);
     const size_t oTri = off;
@@ -101,6 +103,7 @@ template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k,
     ws.W = reinterpret_cast<T *>(base + oW);
     ws.Z = reinterpret_cast<T *>(base + oZ);
     ws.Zt = reinterpret_cast<T *>(base + oZt);
+    ws.Yh = reinterpret_cast<T *>(base + oYh);
     ws.Dwk = reinterpret_cast<T *>(base + oD);
     ws.Tri = reinterpret_cast<T *>(base + oTri);
     ws.tau = reinterpret_cast<T *>(base + oTau);

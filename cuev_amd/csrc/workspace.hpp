@@ -27,6 +27,7 @@ template <typename T> struct Workspace {
     T *W = nullptr;   // n x n: W = Y T per panel (same placement as Y)
     T *Z = nullptr;   // n x k: two-sided companion Z of the current block (ld n)
     T *Zt = nullptr;  // k x n: sbr_back's Y_blk^T M (ld k)
+    T *Yh = nullptr;  // n x k: sbr_back_t's block Y with zeros above each panel (ld n)
     T *Dwk = nullptr; // 2k x b: small products Z^T W, Y^T W, W^T P
     T *Tri = nullptr; // b x b: T factor of the last panel
     T *tau = nullptr; // b: Householder scalars of the last panel
