@@ -213,8 +213,9 @@ __device__ __forceinline__ void bt_load_ops(BtOperands<T, B> &o, const T *__rest
 // columns [0, ncols): one wave per 16 columns (see the header). Each wave keeps the rows of the current pass's window, WS = G*b + 16 rows of
 // its 16 columns, in a private circular LDS buffer (row r in slot r mod WS):
 // the window slides up 16 rows per sweep group, so a group costs 16 rows in and
-// 16 rows out. The operands of the next two blocks and the next group's
-// entering rows are loaded into registers while the current block computes.
+// 16 rows out. The next block's operands and the next group's entering rows
+// are loaded into registers while the current block computes (one block ahead:
+// ~390 VGPRs, so the concurrent D&C kernels still find room on every SIMD).
 // Invalid operands and rows past n are read from zp: zeros past U's last column
 // (the workspace fills ldu*n + 2048 elements of U with zeros and every sweep
 // writes only its own column).
@@ -282,9 +283,8 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
     int top = p.g * NG + 1 + p.h0 * B; // first window row
     BtPos pn = p;
     bool more = TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n);
-    Ops cur, nxt, nx2;
+    Ops cur, nxt;
     bt_load_ops<T, B, TR>(cur, U, ldu, zp, Tf, toff, p.g, p.h, last, q, li);
-    bt_load_ops<T, B, TR>(nxt, U, ldu, zp, Tf, toff, more ? pn.g : p.g, more ? pn.h : p.h, last, q, li);
     for (int c = 0; c < WS; c += 16) {
         T v[4];
         BT_LOAD16(top + c, v);
@@ -292,11 +292,9 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
     }
     T ent[4];
     for (;;) {
-        BtPos pn2 = pn;
-        const bool more2 = more && (TR ? bt_prev<B>(pn2, n) : bt_next<B>(pn2, n));
-        // unconditional (a repeat of this block when nothing follows): no branch
-        bt_load_ops<T, B, TR>(nx2, U, ldu, zp, Tf, toff, more2 ? pn2.g : p.g, more2 ? pn2.h : p.h, last,
-                              q, li);
+        // the next block's operands, in flight during this block's products
+        // (unconditional, a repeat of this block when nothing follows: no branch)
+        bt_load_ops<T, B, TR>(nxt, U, ldu, zp, Tf, toff, more ? pn.g : p.g, more ? pn.h : p.h, last, q, li);
         // rows entering with the next group (the same rows for every block of a group):
         // above the window going down the groups, below it going up (TR)
         BT_LOAD16(TR ? top + WS : top - 16, ent);
@@ -362,10 +360,8 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
             }
         }
         p = pn;
-        pn = pn2;
-        more = more2;
+        more = more && (TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n));
         cur = nxt;
-        nxt = nx2;
     }
 #undef BT_LOAD16
 #undef BT_PUT16

@@ -400,6 +400,7 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     int BMs = (M > 64) ? 128 : 64;
     int BNs = (N > 64) ? 128 : 64;
     if (lower) BMs = BNs = (N >= 1024) ? 128 : 64;
+    if (scratch.small_tiles) BMs = BNs = 64;
     const int cus = device_cu_count();
     auto count_tiles = [&](int bm, int bn) -> i64 {
         const i64 tm = ceil_div(M, bm), tn = ceil_div(N, bn);

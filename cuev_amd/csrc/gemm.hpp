@@ -46,10 +46,13 @@ inline Operand<T> opnd_cat(const T *p0, i64 ld0, const T *p1, i64 ld1, i64 split
     return o;
 }
 
-// Device scratch the engine may use for split-K partial sums.
+// Device scratch the engine may use for split-K partial sums. small_tiles: use
+// only 64 x 64 block tiles (about 100 VGPRs instead of about 220), for products
+// meant to run beside a kernel that holds most of each SIMD's registers.
 struct Scratch {
     void *ptr = nullptr;
     size_t bytes = 0;
+    int small_tiles = 0;
 };
 
 // C <- alpha * op(A) op(B) + beta * C  (lower != 0: only entries row >= col of C

@@ -149,9 +149,13 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
         // A (consumed by the reduction) and evec (written last) serve as the D&C's
         // work buffers. Reordered: the D&C runs on its own stream beside the
         // back-transform still queued on s, without split-K (its scratch is s's).
-        if (reordered)
+        if (reordered) {
+            // small GEMM tiles: the merges then fit beside the back-transform's waves
+            Scratch dcs;
+            dcs.small_tiles = 1;
             CUEV_TRY(dc_solve<T>(*ctx.dc, ctx.dc_stream, n, ctx.h_d, ctx.h_e, ctx.dcQ, n, eval, A, evec,
-                                 ctx.dcQn, Scratch{}, dc_leaf));
+                                 ctx.dcQn, dcs, dc_leaf));
+        }
         else
             CUEV_TRY(dc_solve<T>(*ctx.dc, s, n, ctx.h_d, ctx.h_e, evec, n, eval, A, ws.M, ctx.dcQn, gs,
                                  dc_leaf));
