@@ -83,7 +83,11 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
     double *tauv = sums + 2 * b;                         // [b]
     double *prow = tauv + b;                             // [b] pivot row
     double *red4 = prow + b;                             // [256] partial-sum combine buffer
-    T *P = reinterpret_cast<T *>(red4 + 256);            // [b][CHp] panel rows of this workgroup
+    double *red4b = red4 + 256;                          // [256] second combine buffer
+    double *sx = red4b + 256;                            // [CH] x = column c below the diagonal
+    double *sv = sx + CH;                                // [CH] reflector c-1 (unit at row c-1)
+    double *sjv = sv + CH;                               // [b] tau * v^T a_j of the trailing columns
+    T *P = reinterpret_cast<T *>(sjv + b);               // [b][CHp] panel rows of this workgroup
 
     const int r0g = g * CH;                              // first global panel row owned
     const int chl = max(0, min(CH, a.rows - r0g));       // rows owned
@@ -104,28 +108,43 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
         const int ndot = has_col ? b - c : 0;           // q = 0: ||x||^2, q >= 1: x^T a_{c+q}
         const int ngram = (c >= 2 && c - 1 < a.rows) ? c - 1 : 0; // v_i^T v_{c-1}, i < c-1
 
-        // (1) local partial sums
+        // (1) local partial sums: x^T a_j for j >= c (j = c: ||x||^2) and the Gram
+        // column v_i^T v_{c-1}, i < c-1, from masked copies of x and v_{c-1}. Each
+        // thread owns one column q and a quarter of the rows; no wave reductions.
         double *mypart = a.part + ((size_t)buf * a.G + g) * PS;
         const int nq = ndot + ngram;
-        for (int q = wave; q < nq; q += kPQThreads / 64) {
-            double s = 0.0;
-            if (q < ndot) {
-                const int j = c + q;
-                for (int rl = lane; rl < chl; rl += 64) {
-                    const int r = r0g + rl;
-                    if (r > c) s += (double)P[c * CHp + rl] * (double)P[j * CHp + rl];
-                }
-            } else {
-                const int i = q - ndot; // gram entry (i, c-1)
-                const int cm = c - 1;
-                for (int rl = lane; rl < chl; rl += 64) {
-                    const int r = r0g + rl;
-                    if (r > cm) s += (double)P[i * CHp + rl] * (double)P[cm * CHp + rl];
-                    else if (r == cm) s += (double)P[i * CHp + rl]; // v_{c-1}[c-1] = 1
+        for (int rl = tid; rl < CH; rl += kPQThreads) {
+            const int r = r0g + rl;
+            const bool own = rl < chl;
+            sx[rl] = (own && has_col && r > c) ? (double)P[c * CHp + rl] : 0.0;
+            double v = 0.0;
+            if (own && ngram) v = (r > c - 1) ? (double)P[(c - 1) * CHp + rl] : (r == c - 1 ? 1.0 : 0.0);
+            sv[rl] = v;
+        }
+        __syncthreads();
+        {
+            const int qc = tid & 63, quarter = tid >> 6;
+            const int QR = (CH + 3) >> 2;
+            const int rb = quarter * QR, re = min(chl, rb + QR);
+            double s1 = 0.0, s2 = 0.0;
+            if (qc < b) {
+                const T *pc = P + (size_t)qc * CHp;
+                for (int rl = rb; rl < re; ++rl) {
+                    const double pv = (double)pc[rl];
+                    s1 += pv * sx[rl];
+                    s2 += pv * sv[rl];
                 }
             }
-            s = wave_sum(s);
-            if (lane == 0) mypart[(q < ndot) ? q : b + (q - ndot)] = s;
+            red4[quarter * 64 + qc] = s1;
+            red4b[quarter * 64 + qc] = s2;
+        }
+        __syncthreads();
+        if (tid < b) {
+            const int qc = tid;
+            if (qc >= c && ndot)
+                mypart[qc - c] = ((red4[qc] + red4[64 + qc]) + red4[128 + qc]) + red4[192 + qc];
+            if (qc < ngram)
+                mypart[b + qc] = ((red4b[qc] + red4b[64 + qc]) + red4b[128 + qc]) + red4b[192 + qc];
         }
         // pivot row A[c, c:b] from its owner
         if (has_col && c >= r0g && c < r0g + chl) {
@@ -194,17 +213,22 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
         }
         if (tid == 0) tauv[c] = tau;
 
-        // (4) apply H = I - tau v v^T to columns c+1..b-1 of the owned rows
+        // (4) apply H = I - tau v v^T to columns c+1..b-1 of the owned rows:
+        // a_j -= (tau v^T a_j) v with v^T a_j = a_j[c] + scal * (x^T a_j); each wave
+        // takes every 4th column, each lane fixed rows (no index division)
         const int ntrail = b - c - 1;
         if (tau != 0.0) {
-            for (int idx = tid; idx < ntrail * chl; idx += kPQThreads) {
-                const int jj = idx / chl, rl = idx - jj * chl;
+            for (int jj = tid; jj < ntrail; jj += kPQThreads)
+                sjv[jj] = tau * (prow[c + 1 + jj] + scal * sums[1 + jj]);
+            __syncthreads();
+            for (int rl = lane; rl < chl; rl += 64) {
                 const int r = r0g + rl;
                 if (r < c) continue;
-                const int j = c + 1 + jj;
-                const double sj = prow[j] + scal * sums[1 + jj]; // v^T a_j
                 const double vr = (r == c) ? 1.0 : (double)P[c * CHp + rl] * scal;
-                P[j * CHp + rl] = (T)((double)P[j * CHp + rl] - tau * sj * vr);
+                for (int jj = wave; jj < ntrail; jj += kPQThreads / 64) {
+                    T &e = P[(c + 1 + jj) * CHp + rl];
+                    e = (T)((double)e - sjv[jj] * vr);
+                }
             }
         }
         __syncthreads();
@@ -271,7 +295,8 @@ __global__ void zero_u32_kernel(unsigned *p, int n) {
 }
 
 template <typename T> size_t pq_lds_bytes(int b, int CH) {
-    return sizeof(double) * ((size_t)b * (b + 1) + 2 * b + b + b + 256) + sizeof(T) * (size_t)b * (CH + 1);
+    return sizeof(double) * ((size_t)b * (b + 1) + 2 * b + b + b + 512 + 2 * (size_t)CH + b) +
+           sizeof(T) * (size_t)b * (CH + 1);
 }
 
 } // namespace
