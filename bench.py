@@ -4,9 +4,13 @@
 Default workload: n = 16384, fp64, A = Q diag(lam) Q^T (Q: Householder-QR factor of
 a seeded Gaussian matrix, lam sorted iid U(-1,1)), bandwidth b = 64, block k = 512,
 all eigenvalues and eigenvectors. One step = one full cuev_dsymm_eig_solve call
-(DBBR -> bulge chasing -> host dstedc overlapped with the GPU build of Q_s Q_b ->
-final GEMM), input already resident in HBM; the copy of A that the solver consumes
-is made before each step's timer starts.
+(DBBR -> bulge chasing -> device D&C beside the GPU build of Q_b^T Q_s^T -> final
+GEMM), input already resident in HBM; the copy of A that the solver consumes is
+made before each step's timer starts.
+
+--only chase|copy runs one kernel of interest and nothing else (for hardware counter
+runs): one bulge chase of a random packed band of the configured n and b, or a plain
+1 GiB device copy (the calibration of the FETCH_SIZE / WRITE_SIZE counters).
 
 Prints ONE JSON line: value = mean EVD wall seconds per solve (lower is better),
 per-stage times, correctness against the exact eigenvalues, the roofline of the
@@ -58,6 +62,58 @@ def cpu_baseline(n_sample, n_full, seed=1):
                       f"to n={n_full}"}
 
 
+def measured_traffic(tag, n, b):
+    """HBM bytes per launch of `tag` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE, copy-calibrated), when it was measured for this n and b; else None."""
+    if tag != "bc.chase":
+        return None
+    path = os.path.join(ROOT, "profiles", "bc_chase_q16k_pmc.txt")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    if f"n = {n}, b = {b}" not in text:
+        return None
+    for line in text.splitlines():
+        if line.startswith("# traffic_bytes_per_launch"):
+            return float(line.split()[-1])
+    return None
+
+
+def run_only(which, n, b, k):
+    """One dispatch of interest (after one untimed repeat), for counter collection."""
+    import torch
+    import cuev_amd as cv
+    if which == "copy":
+        src = torch.randn(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"only": "copy", "bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}))
+        return
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    band = torch.randn((n, 2 * b), generator=g, dtype=torch.float64, device="cuda")  # packed, column-major
+    with cv.Handle(torch.float64, n, b, k) as h:
+        d = torch.empty(n, dtype=torch.float64, device="cuda")
+        e = torch.empty(n, dtype=torch.float64, device="cuda")
+        import ctypes
+        fn = cv.lib().cuev_d_bc_chase
+        for _ in range(2):
+            B = band.clone()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = fn(h.h, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(d.data_ptr()),
+                    ctypes.c_void_p(e.data_ptr()))
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            assert st == 0, cv.lib().cuev_last_error_message()
+    hops = n * n / (2.0 * b)
+    print(json.dumps({"only": "chase", "n": n, "b": b, "seconds": dt,
+                      "algorithmic_bytes": hops * 4 * b * b * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +129,8 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="also time the mixed FP32+refinement form")
     ap.add_argument("--cpu-n", type=int, default=6144, help="n of the dsyevd CPU sample (0: skip)")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--only", choices=["chase", "copy"], default=None,
+                    help="run one kernel of interest and exit (hardware counter runs)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -89,6 +147,9 @@ def main():
     n_default = {"q16k": 16384, "n1024": 1024, "g49k": 49152, "mixed16k": 16384}[a.workload]
     n = a.n or n_default
     b, k = a.b, a.k
+    if a.only:
+        run_only(a.only, n, b, k)
+        return
     if a.workload == "n1024" and not a.n:
         b, k = 32, 256
     log(f"[bench] generating workload {a.workload} n={n} on the GPU")
@@ -171,7 +232,8 @@ def main():
             per_launch = r["bytes"] / r["count"]
             ach = per_launch / dur / 1e9
             roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0,
-                        "unit": "GB/s", "frac": round(ach / 8000.0, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(ach / 8000.0, 4),
+                        "traffic": measured_traffic(dom, n, b),
                         "avg_launch_ms": round(dur * 1e3, 3)}
         else:
             per_launch = r["flops"] / r["count"]
