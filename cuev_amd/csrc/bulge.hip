@@ -54,13 +54,13 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
     // (stride WLD doubles across lanes) spreads over the banks
     const int WLD = 2 * b + 1;
     const int NSLOT = 2 * b;     // ring of window columns: matrix column j lives in slot j % NSLOT
-    constexpr int RPL = (3 * B + 63) / 64; // rows of p per lane
-    constexpr int PLD = 64 * RPL;          // stride of the per-wave p partials
     constexpr int NW = kBCThreads / 64;    // waves per workgroup
+    // k segments of the p product: (row, segment) pairs cover the workgroup's threads
+    constexpr int NSEG = (kBCThreads / (3 * B)) < 1 ? 1 : (kBCThreads / (3 * B));
     T *win = reinterpret_cast<T *>(smem_raw);          // [NSLOT][WLD]
     T *sw = win + (size_t)NSLOT * WLD;                 // [b]   reflector w
     T *sp = sw + b;                                    // [3b]  p
-    T *spart = sp + 3 * b;                             // [NW][PLD] partial sums of p
+    T *spart = sp + 3 * b;                             // [NSEG][3b] partial sums of p
     __shared__ T sred[NW];
     __shared__ double s_beta;
     __shared__ int s_go;
@@ -164,31 +164,30 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             }
             __syncthreads();
 
-            // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow: each wave sums an eighth
-            // of the k range for all rows (RPL rows per lane), partials combined below
+            // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow. Thread = (row t, k segment);
+            // each thread walks its segment starting at a row-dependent offset, so the
+            // lanes of a wave (consecutive rows) hit distinct LDS banks both for
+            // A(i, k), i >= k (column k, stride 1 across lanes) and for A(k, i), i < k
+            // (column i: stride WLD - 1 across lanes, which without the rotation puts
+            // every lane on one bank)
             const int nrow = hi - cq + 1; // <= 3b
             {
-                const int kper = (L + NW - 1) / NW;
-                const int k0 = wave * kper, k1 = min(L, k0 + kper);
-                T acc[RPL];
-#pragma unroll
-                for (int r = 0; r < RPL; ++r) acc[r] = T(0);
-#pragma unroll 4
-                for (int q = k0; q < k1; ++q) {
-                    const int k = r0 + q;
-                    const T wk = sw[q];
-#pragma unroll
-                    for (int r = 0; r < RPL; ++r) {
-                        const int i = cq + lane + 64 * r;
-                        if (lane + 64 * r < nrow) {
-                            const T a = (i >= k) ? W_(i, k) : W_(k, i);
-                            acc[r] += a * wk;
-                        }
+                const int t = tid % (3 * B), seg = tid / (3 * B);
+                const int SL = (L + NSEG - 1) / NSEG;
+                const int kb = seg * SL, ke = min(L, kb + SL), len = ke - kb;
+                T acc = T(0);
+                if (seg < NSEG && t < nrow && len > 0) {
+                    const int i = cq + t;
+                    int qq = t % len;
+                    for (int j = 0; j < len; ++j) {
+                        const int q = kb + qq;
+                        const int k = r0 + q;
+                        const T a = (i >= k) ? W_(i, k) : W_(k, i);
+                        acc += a * sw[q];
+                        qq = (qq + 1 == len) ? 0 : qq + 1;
                     }
                 }
-#pragma unroll
-                for (int r = 0; r < RPL; ++r)
-                    if (lane + 64 * r < nrow) spart[wave * PLD + lane + 64 * r] = acc[r];
+                if (seg < NSEG && t < 3 * B) spart[seg * (3 * B) + t] = acc;
             }
             __syncthreads();
             // ---- p, and kappa = w^T p over the reflector's rows [o, o + L)
@@ -198,7 +197,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 if (tid < nrow) {
                     T ps = T(0);
 #pragma unroll
-                    for (int v = 0; v < NW; ++v) ps += spart[v * PLD + tid];
+                    for (int v = 0; v < NSEG; ++v) ps += spart[v * (3 * B) + tid];
                     const T pv = T(2) * ps;
                     sp[tid] = pv;
                     if (tid >= o && tid < o + L) kv = pv * sw[tid - o];
@@ -273,8 +272,8 @@ __global__ void bc_extract_kernel(const T *Bp, T *d, T *e, int n, int b) {
 }
 
 template <typename T> size_t bc_lds_bytes(int b) {
-    const size_t pld = 64 * (size_t)((3 * b + 63) / 64);
-    return sizeof(T) * ((size_t)2 * b * (2 * b + 1) + b + 3 * b + (kBCThreads / 64) * pld);
+    const size_t nseg = std::max<size_t>(1, kBCThreads / (3 * (size_t)b));
+    return sizeof(T) * ((size_t)2 * b * (2 * b + 1) + b + 3 * b + nseg * 3 * b);
 }
 
 } // namespace
