@@ -61,15 +61,26 @@ template <> struct MfmaBt<float> {
     __device__ static __forceinline__ int kidx(int r, int q) { return 4 * q + r; }
 };
 
-// T factor of every block (g, h): one wave per block. V's columns are the
-// hop-h reflectors of sweeps 16g..16g+15 (zero for sweeps past n-3 and for rows
-// past a sweep's last hop, which U holds as zeros).
-template <typename T, int B>
-__global__ __launch_bounds__(kBTThreads) void bt_tfactor_kernel(const T *__restrict__ U, i64 ldu,
-                                                                const i64 *__restrict__ toff,
-                                                                int ngroups, i64 nblocks, int n,
-                                                                T *__restrict__ Tf) {
+// Operands of every block (g, h), packed in the apply kernel's lane order: one
+// wave per block. V's columns are the hop-h reflectors of sweeps 16g..16g+15
+// (zero for sweeps past n-3 and for rows past a sweep's last hop, which U holds
+// as zeros); V[i][j] = U[hb + i - j + (16g + j) ldu] for 0 <= i - j < b. The
+// block is I - V T V^T with T = (I/2 + striu(V^T V))^-1 (reflectors
+// H = I - 2 w w^T); TR packs the transposed block I - V T^T V^T. Per lane
+// (q = lane/16, li = lane%16) the block holds 8 NKB values:
+//   c <  4 NKB: V[16(c/4) + kidx(c%4, q)][li]            (A operand of X = V^T M_J)
+//   c >= 4 NKB: W[16t + li][kidx(r, q)], c = 4 NKB + 4t + r (A operand of M_J += W X)
+// with W = -V T (TR: -V T^T), stored as pairs: value c of lane l at
+// ((c/2) 64 + l) 2 + c%2, so the apply kernel reads each pair with one 16-byte load.
+template <typename T, int B, bool TR>
+__global__ __launch_bounds__(kBTThreads) void bt_pack_kernel(const T *__restrict__ U, i64 ldu,
+                                                             const i64 *__restrict__ toff,
+                                                             int ngroups, i64 nblocks, int n,
+                                                             T *__restrict__ ops) {
+    using MB = MfmaBt<T>;
     constexpr int NG = kBtGroup;
+    constexpr int NKB = B / 16 + 1;
+    constexpr int NOP = 8 * NKB;
     __shared__ T sv[4][NG][B];
     __shared__ T sn[4][NG][NG + 1];
     __shared__ T st[4][NG][NG + 1];
@@ -117,18 +128,29 @@ __global__ __launch_bounds__(kBTThreads) void bt_tfactor_kernel(const T *__restr
         for (int i = j + 1; i < NG; ++i) st[wave][i][j] = T(0);
     }
     __syncthreads();
-    if (act)
-        for (int pidx = lane; pidx < NG * NG; pidx += 64)
-            Tf[id * (NG * NG) + pidx] = st[wave][pidx % NG][pidx / NG];
+    if (!act) return;
+    const int q = lane >> 4, li = lane & 15;
+    T *ob = ops + id * (i64)(NOP * 64) + 2 * lane;
+#pragma unroll
+    for (int c = 0; c < 4 * NKB; ++c) {
+        const int d = 16 * (c >> 2) + MB::kidx(c & 3, q) - li;
+        ob[(c >> 1) * 128 + (c & 1)] = (d >= 0 && d < B) ? sv[wave][li][d] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < NKB; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = MB::kidx(r, q), i = 16 * t + li;
+            T acc = T(0);
+            for (int k = 0; k < NG; ++k) {
+                const int d = i - k;
+                const T v = (d >= 0 && d < B) ? sv[wave][k][d] : T(0);
+                acc += v * (TR ? st[wave][j][k] : st[wave][k][j]);
+            }
+            const int c = 4 * NKB + 4 * t + r;
+            ob[(c >> 1) * 128 + (c & 1)] = -acc;
+        }
 }
-
-// Register set of one block's operands, in the lane layouts of the three products.
-template <typename T, int B> struct BtOperands {
-    static constexpr int NKB = B / 16 + 1;
-    T vx[4 * NKB]; // X = V^T M_J: A operand of chunk c, V[16(c/4) + kidx(c%4, q)][li]
-    T vu[NKB][4];  // M_J -= V Y: A operand of tile t, chunk r, V[16t + li][kidx(r, q)]
-    T t[4];        // Y = T X: A operand of chunk r, T[li][kidx(r, q)]
-};
 
 // Position of a block in the application order (wave-uniform).
 struct BtPos {
@@ -176,74 +198,55 @@ template <int B> __device__ __forceinline__ bool bt_prev(BtPos &p, int n) {
     return true;
 }
 
-// Operands of block (g, h) in the MFMA lane layouts. Invalid entries are read
-// from zp (zeros), so every load is unconditional with no select after it and
-// the loads can stay in flight across the previous blocks' products.
-template <typename T, int B, bool TR>
-__device__ __forceinline__ void bt_load_ops(BtOperands<T, B> &o, const T *__restrict__ U, i64 ldu,
-                                            const T *__restrict__ zp, const T *__restrict__ Tf,
-                                            const i64 *__restrict__ toff, int g, int h, int last,
-                                            int q, int li) {
-    using MB = MfmaBt<T>;
-    constexpr int NKB = B / 16 + 1;
-    const int s0 = g * kBtGroup;
-    const T *Ug = U + (i64)h * B;
+// One block's packed operands (see bt_pack_kernel), as NOP/2 pairs.
+template <typename T, int NP>
+__device__ __forceinline__ void bt_load_ops(T (&o)[2 * NP], const T *__restrict__ ops, i64 id, int lane) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const T2 *p = reinterpret_cast<const T2 *>(ops + id * (i64)(2 * NP * 64)) + lane;
 #pragma unroll
-    for (int c = 0; c < 4 * NKB; ++c) {
-        const int d = 16 * (c >> 2) + MB::kidx(c & 3, q) - li, sw = s0 + li;
-        const bool ok = d >= 0 && d < B && sw <= last;
-        o.vx[c] = *(ok ? Ug + d + (i64)sw * ldu : zp);
+    for (int c = 0; c < NP; ++c) {
+        const T2 v = p[c * 64];
+        o[2 * c] = v.x;
+        o[2 * c + 1] = v.y;
     }
-#pragma unroll
-    for (int t = 0; t < NKB; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int j = MB::kidx(r, q);
-            const int d = 16 * t + li - j, sw = s0 + j;
-            const bool ok = d >= 0 && d < B && sw <= last;
-            o.vu[t][r] = *(ok ? Ug + d + (i64)sw * ldu : zp);
-        }
-    const T *Tb = Tf + (toff[g] + h) * (kBtGroup * kBtGroup);
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        o.t[r] = TR ? Tb[MB::kidx(r, q) + li * kBtGroup] : Tb[li + MB::kidx(r, q) * kBtGroup];
 }
 
-// M <- Q_b M (TR: M <- Q_b^T M, the blocks in reverse order with T^T) for
-// columns [0, ncols): one wave per 16 columns (see the header). Each wave keeps the rows of the current pass's window, WS = G*b + 16 rows of
-// its 16 columns, in a private circular LDS buffer (row r in slot r mod WS):
-// the window slides up 16 rows per sweep group, so a group costs 16 rows in and
-// 16 rows out. The next block's operands and the next group's entering rows
-// are loaded into registers while the current block computes (one block ahead:
-// ~390 VGPRs, so the concurrent D&C kernels still find room on every SIMD).
-// Invalid operands and rows past n are read from zp: zeros past U's last column
-// (the workspace fills ldu*n + 2048 elements of U with zeros and every sweep
-// writes only its own column).
+// M <- Q_b M (TR: M <- Q_b^T M, the blocks in reverse order, each transposed) for
+// columns [0, ncols): one wave per 16 columns (see the header). Each wave keeps the
+// rows of the current pass's window, WS = G*b + 16 rows of its 16 columns, in a
+// private LDS buffer of RS = WS + 64 rows: row r of the matrix sits in buffer row
+// r - org. The window slides 16 rows per sweep group (a group costs 16 rows in and
+// 16 out); when it reaches the end of the buffer, its rows move back to the other
+// end in one copy (every 4 groups), so a block's 16 NKB rows are always contiguous
+// and every LDS access of a block is one base address plus a constant. The next
+// block's operands and the next group's entering rows are loaded into registers
+// while the current block computes. Rows past n are read from zp (zeros) and not
+// written.
 template <typename T, int B, bool TR>
-__global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict__ U, i64 ldu,
-                                                             const T *__restrict__ Tf,
-                                                             const i64 *__restrict__ toff, T *M,
+__global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict__ ops,
+                                                             const i64 *__restrict__ toff,
+                                                             const T *__restrict__ zp, T *M,
                                                              i64 ldm, int n, int ncols) {
     using MB = MfmaBt<T>;
     using V4 = typename MB::v4;
-    using Ops = BtOperands<T, B>;
     constexpr int NG = kBtGroup;
     constexpr int NKB = B / 16 + 1; // 16-row tiles covering a block's b + 15 rows
     constexpr int NC = 4 * NKB;     // 4-row K chunks
+    constexpr int NOP = 2 * NC;     // operand values per lane per block
     constexpr int G = 128 / B;      // hops per pass
     constexpr int WS = G * B + 16;  // window rows
+    constexpr int RS = WS + 64;     // buffer rows
     constexpr int LDW = 17;         // LDS row stride: rows and 16-column reads both conflict-free
-    __shared__ T win[(kBBThreads / 64) * WS * LDW];
+    __shared__ T win[(kBBThreads / 64) * RS * LDW];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, q = lane >> 4, li = lane & 15;
     const int wcol = (blockIdx.x * (kBBThreads / 64) + wave) * 16;
     if (wcol >= ncols) return; // whole wave
-    const int wb = wave * (WS * LDW); // this wave's window
+    const int wb = wave * (RS * LDW); // this wave's buffer
     const int last = n - 3;           // last sweep
-    const T *zp = U + ldu * (i64)n;
-    for (int e = lane; e < WS * LDW; e += 64) win[wb + e] = T(0);
+    for (int e = lane; e < RS * LDW; e += 64) win[wb + e] = T(0);
 
-// rows [r0, r0+16) of the wave's 16 columns: lane = (row li, columns q + 4i)
+// matrix rows [r0, r0+16) of the wave's 16 columns: lane = (row li, columns q + 4i)
 #define BT_LOAD16(r0, v)                                                                           \
     do {                                                                                           \
         const int r_ = (r0) + li;                                                                  \
@@ -253,21 +256,31 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
             (v)[i_] = *(ok_ ? M + r_ + (i64)col_ * ldm : zp);                                      \
         }                                                                                          \
     } while (0)
-#define BT_PUT16(r0, v)                                                                            \
+// buffer rows [b0, b0+16)
+#define BT_PUT16(b0, v)                                                                            \
     do {                                                                                           \
-        const int s_ = ((r0) + li) % WS;                                                           \
-        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[wb + s_ * LDW + q + 4 * i_] = (v)[i_]; \
+        const int o_ = wb + ((b0) + li) * LDW + q;                                                 \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[o_ + 4 * i_] = (v)[i_];               \
     } while (0)
-#define BT_STORE16(r0)                                                                             \
+// matrix rows [r0, r0+16) from buffer rows [b0, b0+16)
+#define BT_STORE16(r0, b0)                                                                         \
     do {                                                                                           \
         const int r_ = (r0) + li;                                                                  \
         if (r_ < n) {                                                                              \
-            const int s_ = r_ % WS;                                                                \
+            const int o_ = wb + ((b0) + li) * LDW + q;                                             \
             _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                     \
                 const int col_ = wcol + q + 4 * i_;                                                \
-                if (col_ < ncols) M[r_ + (i64)col_ * ldm] = win[wb + s_ * LDW + q + 4 * i_];      \
+                if (col_ < ncols) M[r_ + (i64)col_ * ldm] = win[o_ + 4 * i_];                      \
             }                                                                                      \
         }                                                                                          \
+    } while (0)
+// buffer rows [s0, s0+16) -> [d0, d0+16)
+#define BT_MOVE16(s0, d0)                                                                          \
+    do {                                                                                           \
+        T v_[4];                                                                                   \
+        const int os_ = wb + ((s0) + li) * LDW + q, od_ = wb + ((d0) + li) * LDW + q;              \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) v_[i_] = win[os_ + 4 * i_];               \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[od_ + 4 * i_] = v_[i_];               \
     } while (0)
 
     BtPos p;
@@ -280,92 +293,102 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
         p.g = min(last, n - 2) / NG;
         p.h = 0;
     }
+    // the window moves down the buffer going up the groups (TR), up it going down
     int top = p.g * NG + 1 + p.h0 * B; // first window row
+    int org = TR ? top : top - (RS - WS);
     BtPos pn = p;
     bool more = TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n);
-    Ops cur, nxt;
-    bt_load_ops<T, B, TR>(cur, U, ldu, zp, Tf, toff, p.g, p.h, last, q, li);
+    T cur[NOP], nxt[NOP];
+    bt_load_ops<T, NC>(cur, ops, toff[p.g] + p.h, lane);
     for (int c = 0; c < WS; c += 16) {
         T v[4];
         BT_LOAD16(top + c, v);
-        BT_PUT16(top + c, v);
+        BT_PUT16(top + c - org, v);
     }
+    // this lane's first element of a block tile: row kidx(0, q), column li; the
+    // others are at constant offsets (kidx(r, q) = kidx(0, q) + kidx(r, 0))
+    const int lofs = wb + MB::kidx(0, q) * LDW + li;
     T ent[4];
     for (;;) {
         // the next block's operands, in flight during this block's products
         // (unconditional, a repeat of this block when nothing follows: no branch)
-        bt_load_ops<T, B, TR>(nxt, U, ldu, zp, Tf, toff, more ? pn.g : p.g, more ? pn.h : p.h, last, q, li);
+        bt_load_ops<T, NC>(nxt, ops, more ? toff[pn.g] + pn.h : toff[p.g] + p.h, lane);
         // rows entering with the next group (the same rows for every block of a group):
         // above the window going down the groups, below it going up (TR)
         BT_LOAD16(TR ? top + WS : top - 16, ent);
 
         // ---- block (g, h): rows [base, base + 16 NKB) of the window
         const int base = p.g * NG + 1 + p.h * B;
-        const int sb = base % WS;
+        const int wr = lofs + (base - org) * LDW;
         T m[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            int s = sb + 16 * (c >> 2) + MB::kidx(c & 3, q);
-            if (s >= WS) s -= WS;
-            m[c] = win[wb + s * LDW + li];
-        }
+        for (int c = 0; c < NC; ++c) m[c] = win[wr + (16 * (c >> 2) + MB::kidx(c & 3, 0)) * LDW];
         // X = V^T M_J in two independent chains (even and odd chunks)
         V4 x = {T(0), T(0), T(0), T(0)}, x1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int c = 0; c < NC; c += 2) {
-            x = MB::mma(cur.vx[c], m[c], x);
-            x1 = MB::mma(cur.vx[c + 1], m[c + 1], x1);
+            x = MB::mma(cur[c], m[c], x);
+            x1 = MB::mma(cur[c + 1], m[c + 1], x1);
         }
         x += x1;
-        V4 y = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) y = MB::mma(cur.t[r], x[r], y);
-        y = -y; // M_J += V (-Y)
+        // M_J += W X, W = -V T (or -V T^T): the registers of X are the B operand
 #pragma unroll
         for (int t = 0; t < NKB; ++t) {
             V4 acc = {m[4 * t], m[4 * t + 1], m[4 * t + 2], m[4 * t + 3]};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc = MB::mma(cur.vu[t][r], y[r], acc);
+            for (int r = 0; r < 4; ++r) acc = MB::mma(cur[NC + 4 * t + r], x[r], acc);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int s = sb + 16 * t + MB::kidx(r, q);
-                if (s >= WS) s -= WS;
-                win[wb + s * LDW + li] = acc[r];
-            }
+            for (int r = 0; r < 4; ++r) win[wr + (16 * t + MB::kidx(r, 0)) * LDW] = acc[r];
         }
 
         // ---- window bookkeeping for the next block
         if (!more || pn.h0 != p.h0) {
             // pass done: write the window back
-            for (int c = 0; c < WS; c += 16) BT_STORE16(top + c);
+            for (int c = 0; c < WS; c += 16) BT_STORE16(top + c, top + c - org);
             if (!more) break;
             wave_stores_done(); // the next pass reads these rows back through other lanes
             top = pn.g * NG + 1 + pn.h0 * B;
+            org = TR ? top : top - (RS - WS);
             for (int c = 0; c < WS; c += 16) {
                 T v[4];
                 BT_LOAD16(top + c, v);
-                BT_PUT16(top + c, v);
+                BT_PUT16(top + c - org, v);
             }
         } else if (pn.g != p.g) {
             if (TR) {
                 // group g -> g+1: 16 rows leave at the top, 16 enter at the bottom
-                BT_STORE16(top);
-                BT_PUT16(top + WS, ent);
+                BT_STORE16(top, top - org);
                 top += 16;
+                if (top + WS - org > RS) {
+                    // rows [top, top + WS - 16) back to the start of the buffer (moving
+                    // down by >= 16 rows: lowest first)
+                    for (int c = 0; c < WS - 16; c += 16) BT_MOVE16(top - org + c, c);
+                    org = top;
+                }
+                BT_PUT16(top + WS - 16 - org, ent);
             } else {
                 // group g -> g-1: 16 rows leave at the bottom, 16 enter at the top
-                BT_STORE16(top + WS - 16);
-                BT_PUT16(top - 16, ent);
+                BT_STORE16(top + WS - 16, top + WS - 16 - org);
                 top -= 16;
+                if (top < org) {
+                    // rows [top + 16, top + WS) to the end of the buffer (moving up by
+                    // >= 16 rows: highest first)
+                    const int norg = top - (RS - WS);
+                    for (int c = WS - 32; c >= 0; c -= 16) BT_MOVE16(top + 16 - org + c, top + 16 - norg + c);
+                    org = norg;
+                }
+                BT_PUT16(top - org, ent);
             }
         }
         p = pn;
         more = more && (TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n));
-        cur = nxt;
+#pragma unroll
+        for (int c = 0; c < NOP; ++c) cur[c] = nxt[c];
     }
 #undef BT_LOAD16
 #undef BT_PUT16
 #undef BT_STORE16
+#undef BT_MOVE16
 }
 
 template <typename T, int B>
@@ -373,21 +396,26 @@ int launch_bc_back(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, b
     const int n = ws.n;
     if (ws.bt_blocks <= 0) return CUEV_STATUS_SUCCESS;
     {
-        prof::Scope sc(s, "bt.bc_back_t", (double)ws.bt_blocks * kBtGroup * kBtGroup * B);
-        hipLaunchKernelGGL((bt_tfactor_kernel<T, B>), dim3((unsigned)ceil_div(ws.bt_blocks, 4)),
-                           dim3(kBTThreads), 0, s, U, ws.ldu, ws.bt_off, ws.bt_groups, ws.bt_blocks, n,
-                           ws.BtT);
+        prof::Scope sc(s, "bt.bc_pack", (double)ws.bt_blocks * kBtGroup * kBtGroup * B);
+        const dim3 grid((unsigned)ceil_div(ws.bt_blocks, 4));
+        if (tr)
+            hipLaunchKernelGGL((bt_pack_kernel<T, B, true>), grid, dim3(kBTThreads), 0, s, U, ws.ldu,
+                               ws.bt_off, ws.bt_groups, ws.bt_blocks, n, ws.BtT);
+        else
+            hipLaunchKernelGGL((bt_pack_kernel<T, B, false>), grid, dim3(kBTThreads), 0, s, U, ws.ldu,
+                               ws.bt_off, ws.bt_groups, ws.bt_blocks, n, ws.BtT);
         CUEV_LAUNCH_CHECK();
     }
     // 4 flops per reflector entry per column: sum_s (n-1-s) entries ~ n^2/2
     prof::Scope sc(s, "bt.bc_back", 2.0 * (double)n * n * n);
     const i64 blocks = ceil_div(n, 16 * (kBBThreads / 64));
+    const T *zp = U + ws.ldu * (i64)n; // zeros past U's last column
     if (tr)
-        hipLaunchKernelGGL((bc_back_kernel<T, B, true>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s, U,
-                           ws.ldu, ws.BtT, ws.bt_off, M, ldm, n, n);
+        hipLaunchKernelGGL((bc_back_kernel<T, B, true>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
+                           ws.BtT, ws.bt_off, zp, M, ldm, n, n);
     else
         hipLaunchKernelGGL((bc_back_kernel<T, B, false>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
-                           U, ws.ldu, ws.BtT, ws.bt_off, M, ldm, n, n);
+                           ws.BtT, ws.bt_off, zp, M, ldm, n, n);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
 }

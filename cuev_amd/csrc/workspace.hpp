@@ -44,8 +44,9 @@ template <typename T> struct Workspace {
     int *prog = nullptr;         // bc_chase per-sweep progress (n ints)
     int *err = nullptr;          // device error flag
     // bc_back block reflectors: sweeps grouped kBtGroup at a time; block (g, h) holds
-    // the hop-h reflectors of sweeps [16g, 16g+16) as I - V T V^T. T factors are
-    // kBtGroup^2 each, block (g, h) at index bt_off[g] + h.
+    // the hop-h reflectors of sweeps [16g, 16g+16) as I - V T V^T, its operands packed
+    // in the apply kernel's register order (bt_pack_values(b) values), block (g, h)
+    // at index bt_off[g] + h.
     T *BtT = nullptr;
     i64 *bt_off = nullptr; // bt_groups + 1 prefix offsets
     int bt_groups = 0;
@@ -56,6 +57,8 @@ template <typename T> struct Workspace {
 };
 
 constexpr int kBtGroup = 16; // sweeps per bc_back block reflector
+// values per packed bc_back block: 64 lanes x (b/16 + 1) 16-row tiles x 8 (V and -V T)
+constexpr int bt_pack_values(int b) { return 64 * 8 * (b / 16 + 1); }
 
 template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k, hipStream_t s);
 template <typename T> void workspace_free(Workspace<T> &ws);
