@@ -30,6 +30,7 @@ namespace cuev {
 namespace {
 
 constexpr int kPQThreads = 256;
+constexpr size_t kPQGateBytes = 256; // scratch head: the CholQR -> Householder gate word
 
 template <typename T> struct PQArgs {
     T *A;
@@ -43,6 +44,7 @@ template <typename T> struct PQArgs {
     int rows, b, CH, G;
     double *part;   // [2][G][2b]
     double *pivot;  // [2][b]
+    const int *gate; // when set: run only if *gate != 0 (the CholQR kernel declined the panel)
     unsigned *counter;
     int *err;
 };
@@ -73,6 +75,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned *counter, unsigned target,
 
 template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_kernel(PQArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    if (a.gate && *a.gate == 0) return;
     const int b = a.b, CH = a.CH, CHp = CH + 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = blockIdx.x;
@@ -289,6 +292,533 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
     }
 }
 
+// ---------------------------------------------------------------------------
+// CholQR2 panel factorization with Householder reconstruction (the fast path).
+//
+// Two passes of Cholesky QR make an orthonormal Q with A = Q R (each pass: the
+// Gram matrix P^T P of the panel, its Cholesky factor R_p and P <- P R_p^-1, so
+// R = R_2 R_1). The Householder form of the same factorization follows from an
+// LU factorization without pivoting of the top b x b block of Q with signs
+// S = diag(s_k), s_k = -sign of the k-th pivot before the shift:
+//   Q_1 - S = L U,  Y = [L; Q_2 U^-1],  T = -U S L^-T,  R_house = S R,
+// so that (I - Y T Y^T)[:, :b] = Q S (Ballard, Demmel, Grigori, Jacquelin, Knight,
+// Nguyen, "Reconstructing Householder vectors from tall-skinny QR", J. Parallel
+// Distrib. Comput. 85, 2015). In exact arithmetic these are the reflectors,
+// T and R of Householder QR on the panel itself, the LAPACK convention above.
+// CholQR2 keeps Q orthonormal to working precision when cond(panel) is well
+// below eps^-1/2; a panel whose first Gram pivots reveal a (nearly) dependent
+// column (d_k <= 1e-10 G_kk, or not finite) is left untouched and flagged for
+// the Householder kernel, which then runs in full (*gate = 1).
+//
+// Parallel structure: G workgroups (one per CU), each with CH rows in LDS.
+// Per pass: partial Gram -> global, grid barrier, each workgroup sums a slice of
+// the entries over all partials in a fixed order, grid barrier, then every
+// workgroup factors the same b x b Gram matrix redundantly (bit-identical
+// results everywhere). Workgroup 0 then factors Q_1 and publishes U^-1 and T
+// (fifth barrier). The b x b factorizations keep the matrix in registers as 4x4
+// blocks per thread and broadcast the pivot column/row through LDS: one LDS
+// barrier per elimination step. All arithmetic is FP64.
+constexpr int kCQThreads = 256;
+constexpr int kCQMaxCH = 96;
+
+template <typename T> struct CQArgs {
+    T *A;
+    i64 lda;
+    T *Y;
+    i64 ldy;
+    T *W;
+    i64 ldw;
+    T *Tm;
+    T *tau;
+    int rows, CH, G;
+    double *part; // [G][b*b] partial Gram matrices
+    double *gsum; // [b*b]
+    double *bc;   // [2*b*b]: U^-1, then T (column-major)
+    int *gate;
+    unsigned *counter;
+    int *err;
+};
+
+// out = P^T P over the first nr rows (P column-major, ld CHp); thread = 4x4 block.
+template <int B>
+__device__ __forceinline__ void cq_gram(const double *P, int CHp, int nr, double *out, int tid) {
+    constexpr int NB = B / 4;
+    if (tid >= NB * NB) return;
+    const int ti = tid % NB, tj = tid / NB;
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    const double *pa = P + (size_t)(4 * ti) * CHp, *pb = P + (size_t)(4 * tj) * CHp;
+    for (int r = 0; r < nr; ++r) {
+        double x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            x[u] = pa[u * CHp + r];
+            y[u] = pb[u * CHp + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * y[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[(4 * ti + u) + (size_t)(4 * tj + v) * B] = acc[u][v];
+}
+
+// P[rows r0..r1) <- P U for an upper-triangular U (LDS, column-major, ld LD), in
+// place: rounds of whole 4-row blocks, all column blocks of a row block in the
+// same round (a row's new values depend on that row only).
+template <int B>
+__device__ __forceinline__ void cq_trmm(double *P, int CHp, int r0, int r1, const double *U, int tid) {
+    constexpr int NB = B / 4, LD = B + 1;
+    constexpr int RB_PER_ROUND = kCQThreads / NB;
+    const int nrb = (r1 - r0 + 3) / 4;
+    for (int rb0 = 0; rb0 < nrb; rb0 += RB_PER_ROUND) {
+        const int rb = rb0 + tid / NB, tj = tid % NB;
+        const bool act = rb < nrb;
+        double acc[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+        const int rr = r0 + 4 * rb;
+        if (act) {
+            for (int i = 0; i < 4 * tj + 4; ++i) {
+                double x[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = P[(size_t)i * CHp + rr + u];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) w[v] = U[i + (4 * tj + v) * LD];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * w[v];
+            }
+        }
+        __syncthreads();
+        if (act)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (rr + u < r1) P[(size_t)(4 * tj + v) * CHp + rr + u] = acc[u][v];
+        __syncthreads();
+    }
+}
+
+template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel_cholqr_kernel(CQArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int NB = B / 4, LD = B + 1, BB = B * B;
+    const int CH = a.CH, CHp = CH + 1;
+    const int tid = threadIdx.x, g = blockIdx.x;
+    const bool own = tid < NB * NB;
+    const int ti = own ? tid % NB : 0, tj = own ? tid / NB : 0;
+    double *P = reinterpret_cast<double *>(smem_raw); // [B][CHp]: panel(r, c) at P[c*CHp + r]
+    double *M1 = P + (size_t)B * CHp;                 // [B][LD] column-major
+    double *M2 = M1 + B * LD;
+    double *M3 = M2 + B * LD;
+    double *cb = M3 + B * LD; // [2][B] broadcast: pivot column
+    double *rb = cb + 2 * B;  // [2][B] broadcast: pivot row
+    double *xb = rb + 2 * B;  // [2][B] broadcast: row of the inverse
+    double *dv = xb + 2 * B;  // [B] pivots
+    double *gd = dv + B;      // [B] Gram diagonal before the factorization
+    double *sv = gd + B;      // [B] signs S
+    double *red = sv + B;     // [kCQThreads]
+
+    const int r0g = g * CH;
+    const int chl = max(0, min(CH, a.rows - r0g));
+    for (int idx = tid; idx < B * CH; idx += kCQThreads) {
+        const int j = idx / CH, rl = idx - j * CH;
+        P[(size_t)j * CHp + rl] = (rl < chl) ? (double)a.A[(i64)(r0g + rl) + (i64)j * a.lda] : 0.0;
+    }
+    __syncthreads();
+
+    unsigned nbar = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        // ---- Gram matrix of the panel: partials, then fixed-order slice sums
+        cq_gram<B>(P, CHp, chl, a.part + (size_t)g * BB, tid);
+        if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
+        {
+            const int S = (BB + a.G - 1) / a.G;
+            const int e0 = g * S, ne = max(0, min(BB, e0 + S) - e0);
+            int tpe = 1; // threads per entry (power of two)
+            while (tpe < 64 && tpe * 2 * max(ne, 1) <= kCQThreads) tpe *= 2;
+            const int per_round = kCQThreads / tpe;
+            for (int base = 0; base < ne; base += per_round) {
+                const int el = base + tid / tpe, pt = tid % tpe;
+                double s = 0.0;
+                if (el < ne) {
+                    const double *src = a.part + e0 + el;
+                    for (int g0 = pt; g0 < a.G; g0 += tpe * 16) {
+                        double v[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) {
+                            const int gg = g0 + u * tpe;
+                            v[u] = (gg < a.G) ? src[(size_t)gg * BB] : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) s += v[u];
+                    }
+                }
+                red[tid] = s;
+                __syncthreads();
+                if (el < ne && pt == 0) {
+                    double t = 0.0;
+                    for (int q = 0; q < tpe; ++q) t += red[tid + q];
+                    a.gsum[e0 + el] = t;
+                }
+                __syncthreads();
+            }
+        }
+        if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
+
+        // ---- G = L' D L'^T with X = L'^-1 (registers), R_p = D^1/2 L'^T
+        double gr[4][4], xr[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                gr[u][v] = own ? a.gsum[(4 * ti + u) + (size_t)(4 * tj + v) * B] : 0.0;
+                xr[u][v] = (4 * ti + u == 4 * tj + v) ? 1.0 : 0.0;
+            }
+        double *Rout = (pass == 0) ? M1 : M3;
+        if (tid < B) gd[tid] = a.gsum[tid + (size_t)tid * B];
+        if (own && tj == 0)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cb[4 * ti + u] = gr[u][0];
+        if (own && ti == 0)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) xb[4 * tj + v] = xr[0][v];
+        const double tol = (pass == 0) ? 1e-10 : 1e-3;
+        bool ok = true;
+        for (int kb = 0; kb < NB && ok; ++kb) {
+#pragma unroll
+            for (int v0 = 0; v0 < 4; ++v0) {
+                const int k = 4 * kb + v0, p = k & 1;
+                __syncthreads();
+                const double dk = cb[p * B + k];
+                if (!(dk > tol * gd[k]) || !(dk < 1e300)) {
+                    ok = false;
+                    break;
+                }
+                const double rk = 1.0 / dk, rs = 1.0 / sqrt(dk);
+                double ci[4], cj[4], xk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ci[u] = cb[p * B + 4 * ti + u];
+                    cj[u] = cb[p * B + 4 * tj + u];
+                    xk[u] = xb[p * B + 4 * tj + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = 4 * ti + u, j = 4 * tj + v;
+                        if (i > k && j > k) gr[u][v] -= (ci[u] * cj[v]) * rk;
+                        if (i > k && j <= k) xr[u][v] -= (ci[u] * rk) * xk[v];
+                    }
+                if (tid == 0) dv[k] = dk;
+                if (own && tj == 0)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = 4 * ti + u;
+                        Rout[k + i * LD] = (i > k) ? ci[u] * rs : (i == k ? sqrt(dk) : 0.0);
+                    }
+                const int kn = k + 1, vn = (v0 + 1) & 3, bn = kn >> 2;
+                if (kn < B) {
+                    if (own && tj == bn)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) cb[(kn & 1) * B + 4 * ti + u] = gr[u][vn];
+                    if (own && ti == bn)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) xb[(kn & 1) * B + 4 * tj + v] = xr[vn][v];
+                }
+            }
+        }
+        if (!ok) {
+            if (g == 0 && tid == 0) *a.gate = 1;
+            return; // every workgroup takes the same decision
+        }
+        // R_p^-1 (i, j) = X(j, i) / sqrt(d_j) into M2
+        __syncthreads();
+        if (own)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    M2[(4 * tj + v) + (4 * ti + u) * LD] = xr[u][v] / sqrt(dv[4 * ti + u]);
+        __syncthreads();
+        cq_trmm<B>(P, CHp, 0, CH, M2, tid);
+        if (pass == 1) {
+            // R = R_2 R_1 into M1
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+            if (own)
+                for (int k = 0; k < B; ++k) {
+                    double x[4], w[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        x[u] = M3[(4 * ti + u) + k * LD];
+                        w[u] = M1[k + (4 * tj + u) * LD];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * w[v];
+                }
+            __syncthreads();
+            if (own)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) M1[(4 * ti + u) + (4 * tj + v) * LD] = acc[u][v];
+            __syncthreads();
+        }
+    }
+
+    // ---- workgroup 0: Q_1 - S = L U (signs chosen on the way), L^-1, T, U^-1
+    if (g == 0) {
+        double mr[4][4], xr[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                mr[u][v] = own ? P[(size_t)(4 * tj + v) * CHp + 4 * ti + u] : 0.0;
+                xr[u][v] = (4 * ti + u == 4 * tj + v) ? 1.0 : 0.0;
+            }
+        if (own && tj == 0)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cb[4 * ti + u] = mr[u][0];
+        if (own && ti == 0)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                rb[4 * tj + v] = mr[0][v];
+                xb[4 * tj + v] = xr[0][v];
+            }
+        double *Lm = M2, *Um = M3; // unit-lower L, upper U (column-major)
+        for (int kb = 0; kb < NB; ++kb) {
+#pragma unroll
+            for (int v0 = 0; v0 < 4; ++v0) {
+                const int k = 4 * kb + v0, p = k & 1;
+                __syncthreads();
+                const double mkk = cb[p * B + k];
+                const double s = (mkk >= 0.0) ? -1.0 : 1.0;
+                const double piv = mkk - s, rp = 1.0 / piv;
+                double li[4], rj[4], xk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    li[u] = cb[p * B + 4 * ti + u] * rp;
+                    rj[u] = rb[p * B + 4 * tj + u];
+                    xk[u] = xb[p * B + 4 * tj + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = 4 * ti + u, j = 4 * tj + v;
+                        if (i > k && j > k) mr[u][v] -= li[u] * rj[v];
+                        if (i > k && j <= k) xr[u][v] -= li[u] * xk[v];
+                    }
+                if (tid == 0) {
+                    sv[k] = s;
+                    dv[k] = piv;
+                }
+                if (own && tj == 0)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = 4 * ti + u;
+                        Lm[i + k * LD] = (i > k) ? li[u] : (i == k ? 1.0 : 0.0);
+                    }
+                if (own && ti == 0)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int j = 4 * tj + v;
+                        Um[k + j * LD] = (j > k) ? rj[v] : (j == k ? piv : 0.0);
+                    }
+                const int kn = k + 1, vn = (v0 + 1) & 3, bn = kn >> 2;
+                if (kn < B) {
+                    if (own && tj == bn)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) cb[(kn & 1) * B + 4 * ti + u] = mr[u][vn];
+                    if (own && ti == bn)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            rb[(kn & 1) * B + 4 * tj + v] = mr[vn][v];
+                            xb[(kn & 1) * B + 4 * tj + v] = xr[vn][v];
+                        }
+                }
+            }
+        }
+        __syncthreads();
+        // panel rows 0..b-1 of A: R_house = S R on and above the diagonal, L below
+        for (int idx = tid; idx < BB; idx += kCQThreads) {
+            const int r = idx % B, j = idx / B;
+            const double v = (j >= r) ? sv[r] * M1[r + j * LD] : Lm[r + j * LD];
+            a.A[(i64)r + (i64)j * a.lda] = (T)v;
+            if (a.Y) a.Y[(i64)r + (i64)j * a.ldy] = (T)((j < r) ? Lm[r + j * LD] : (j == r ? 1.0 : 0.0));
+        }
+        // L^-1 into P rows 0..b-1 (Q_1 is no longer needed there)
+        if (own)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) P[(size_t)(4 * tj + v) * CHp + 4 * ti + u] = xr[u][v];
+        __syncthreads();
+        // T = -U S L^-T  (T(i, j) = -sum_k U(i, k) s_k Linv(j, k)) into M1
+        {
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+            if (own)
+                for (int k = 0; k < B; ++k) {
+                    const double sk = sv[k];
+                    double x[4], w[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        x[u] = Um[(4 * ti + u) + k * LD] * sk;
+                        w[u] = P[(size_t)k * CHp + 4 * tj + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[u][v] -= x[u] * w[v];
+                }
+            if (own)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) M1[(4 * ti + u) + (4 * tj + v) * LD] = acc[u][v];
+        }
+        // U^-1 by back substitution, one row per step: rhs rows i < k -= U(i, k) V(k, :)
+        double vr[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) vr[u][v] = (4 * ti + u == 4 * tj + v) ? 1.0 : 0.0;
+        if (own && ti == NB - 1)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                vr[3][v] /= Um[(B - 1) + (B - 1) * LD];
+                xb[((B - 1) & 1) * B + 4 * tj + v] = vr[3][v];
+            }
+        for (int kb = NB - 1; kb >= 0; --kb) {
+#pragma unroll
+            for (int v0 = 3; v0 >= 0; --v0) {
+                const int k = 4 * kb + v0, p = k & 1;
+                if (k == 0) break;
+                __syncthreads();
+                double uk[4], vk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    uk[u] = Um[(4 * ti + u) + k * LD];
+                    vk[u] = xb[p * B + 4 * tj + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (4 * ti + u < k) vr[u][v] -= uk[u] * vk[v];
+                const int kp = k - 1, up = (v0 + 3) & 3, bp = kp >> 2;
+                if (own && ti == bp) {
+                    const double d = Um[kp + kp * LD];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        vr[up][v] /= d;
+                        xb[(kp & 1) * B + 4 * tj + v] = vr[up][v];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (own)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 4 * ti + u, j = 4 * tj + v;
+                    M3[i + j * LD] = vr[u][v];
+                    a.bc[i + (size_t)j * B] = vr[u][v];
+                    a.bc[BB + i + (size_t)j * B] = M1[i + j * LD];
+                }
+        // Y rows 0..b-1 into P (unit lower L)
+        for (int idx = tid; idx < BB; idx += kCQThreads) {
+            const int r = idx % B, j = idx / B;
+            P[(size_t)j * CHp + r] = (j < r) ? Lm[r + j * LD] : (j == r ? 1.0 : 0.0);
+        }
+        if (tid == 0) *a.gate = 0;
+    }
+    if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
+    if (g != 0) {
+        for (int idx = tid; idx < BB; idx += kCQThreads) {
+            const int i = idx % B, j = idx / B;
+            M3[i + j * LD] = a.bc[idx];
+            M1[i + j * LD] = a.bc[BB + idx];
+        }
+    }
+    __syncthreads();
+    // Y = Q U^-1 for the rows below the first b
+    const int ylo = (g == 0) ? B : 0;
+    cq_trmm<B>(P, CHp, ylo, CH, M3, tid);
+    // outputs: Y, the reflector tails in A, and W = Y T
+    for (int idx = tid; idx < B * chl; idx += kCQThreads) {
+        const int j = idx / chl, rl = idx - j * chl;
+        if (rl < ylo) continue;
+        const T y = (T)P[(size_t)j * CHp + rl];
+        const i64 r = r0g + rl;
+        a.A[r + (i64)j * a.lda] = y;
+        if (a.Y) a.Y[r + (i64)j * a.ldy] = y;
+    }
+    if (a.W) {
+        const int nrb = (chl + 3) / 4;
+        for (int blk = tid; blk < nrb * NB; blk += kCQThreads) {
+            const int rbk = blk / NB, cj = blk % NB;
+            const int rr = 4 * rbk;
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+            for (int i = 0; i < 4 * cj + 4; ++i) {
+                double x[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = P[(size_t)i * CHp + rr + u];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) w[v] = M1[i + (4 * cj + v) * LD];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * w[v];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (rr + u < chl)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        a.W[(i64)(r0g + rr + u) + (i64)(4 * cj + v) * a.ldw] = (T)acc[u][v];
+        }
+    }
+    if (g == 0) {
+        for (int idx = tid; idx < BB; idx += kCQThreads) {
+            const int i = idx % B, j = idx / B;
+            a.Tm[idx] = (T)M1[i + j * LD];
+        }
+        for (int j = tid; j < B; j += kCQThreads) a.tau[j] = (T)M1[j + j * LD];
+    }
+}
+
+template <int B> size_t cq_lds_bytes(int CH) {
+    return sizeof(double) * ((size_t)B * (CH + 1) + 3 * (size_t)B * (B + 1) + 9 * (size_t)B + kCQThreads);
+}
+
 __global__ void zero_u32_kernel(unsigned *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -310,16 +840,82 @@ template <typename T> int panel_qr_chunk(int b) {
 }
 
 size_t panel_qr_scratch_bytes(int n, int b) {
-    // partials [2][G][2b] doubles + pivot [2][b] doubles, G <= n / 64 + 1
+    // gate word; Householder: partials [2][G][2b] + pivot [2][b] doubles, G <= n / 64 + 2;
+    // CholQR: partial Gram matrices [G][b*b] + sums [b*b] + broadcast [2*b*b], G <= n / 64 + 1
     const size_t G = (size_t)n / 64 + 2;
-    return sizeof(double) * (2 * G * 2 * b + 2 * b) + 256;
+    const size_t hh = sizeof(double) * (2 * G * 2 * b + 2 * b);
+    const size_t cq = sizeof(double) * ((G + 3) * (size_t)b * b);
+    return kPQGateBytes + std::max(hh, cq) + 256;
 }
+
+namespace {
+
+// CholQR2 launch if the panel qualifies: b in {16, 32, 64}, at least 2b rows, and
+// rows spread one workgroup per CU with at most kCQMaxCH rows each.
+template <typename T, int B>
+int launch_cholqr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T *Tm, T *tau, int rows,
+                  void *scratch, size_t scratch_bytes, unsigned *counter, int *err, bool *launched) {
+    *launched = false;
+    const int ncu = device_cu_count();
+    const int CH = std::max(64, (int)round_up(ceil_div(rows, ncu), 16));
+    if (CH > kCQMaxCH) return CUEV_STATUS_SUCCESS;
+    const int G = (int)ceil_div(rows, CH);
+    const size_t lds = cq_lds_bytes<B>(CH);
+    CUEV_HIP(hipFuncSetAttribute((const void *)panel_cholqr_kernel<T, B>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, panel_cholqr_kernel<T, B>, kCQThreads, lds));
+    if (G > per_cu * ncu) return CUEV_STATUS_SUCCESS;
+    const size_t need = kPQGateBytes + sizeof(double) * ((size_t)(G + 3) * B * B);
+    if (need > scratch_bytes) return CUEV_STATUS_SUCCESS;
+    CQArgs<T> a;
+    a.A = A;
+    a.lda = lda;
+    a.Y = Y;
+    a.ldy = ldy;
+    a.W = W;
+    a.ldw = ldw;
+    a.Tm = Tm;
+    a.tau = tau;
+    a.rows = rows;
+    a.CH = CH;
+    a.G = G;
+    a.gate = static_cast<int *>(scratch);
+    a.part = reinterpret_cast<double *>(static_cast<unsigned char *>(scratch) + kPQGateBytes);
+    a.gsum = a.part + (size_t)G * B * B;
+    a.bc = a.gsum + B * B;
+    a.counter = counter;
+    a.err = err;
+    hipLaunchKernelGGL((panel_cholqr_kernel<T, B>), dim3(G), dim3(kCQThreads), lds, s, a);
+    CUEV_LAUNCH_CHECK();
+    *launched = true;
+    return CUEV_STATUS_SUCCESS;
+}
+
+} // namespace
 
 template <typename T>
 int panel_qr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T *Tm, T *tau, int rows,
              int b, void *scratch, size_t scratch_bytes, unsigned *counter, int *err) {
     if (rows < 1 || b < 1 || b > 64)
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "panel_qr: rows=%d b=%d", rows, b);
+    if (scratch_bytes < kPQGateBytes)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "panel_qr: scratch too small");
+    // counter[0]: CholQR barriers, counter[1]: Householder barriers
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, counter, 2);
+    CUEV_LAUNCH_CHECK();
+    bool cq = false;
+    if (rows >= 2 * b) {
+        if (b == 64)
+            CUEV_TRY((launch_cholqr<T, 64>(s, A, lda, Y, ldy, W, ldw, Tm, tau, rows, scratch, scratch_bytes,
+                                           counter, err, &cq)));
+        else if (b == 32)
+            CUEV_TRY((launch_cholqr<T, 32>(s, A, lda, Y, ldy, W, ldw, Tm, tau, rows, scratch, scratch_bytes,
+                                           counter, err, &cq)));
+        else if (b == 16)
+            CUEV_TRY((launch_cholqr<T, 16>(s, A, lda, Y, ldy, W, ldw, Tm, tau, rows, scratch, scratch_bytes,
+                                           counter, err, &cq)));
+    }
     const int CH = panel_qr_chunk<T>(b);
     const int G = (int)ceil_div(rows, CH);
     const size_t lds = pq_lds_bytes<T>(b, CH);
@@ -334,7 +930,7 @@ int panel_qr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T *Tm, 
         CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED,
                   "panel_qr: %d rows need %d co-resident workgroups, device holds %d", rows, G,
                   resident);
-    const size_t need = sizeof(double) * ((size_t)2 * G * 2 * b + 2 * b);
+    const size_t need = kPQGateBytes + sizeof(double) * ((size_t)2 * G * 2 * b + 2 * b);
     if (need > scratch_bytes)
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "panel_qr: scratch too small (%zu < %zu)",
                   scratch_bytes, need);
@@ -351,12 +947,11 @@ int panel_qr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T *Tm, 
     a.b = b;
     a.CH = CH;
     a.G = G;
-    a.part = static_cast<double *>(scratch);
+    a.part = reinterpret_cast<double *>(static_cast<unsigned char *>(scratch) + kPQGateBytes);
     a.pivot = a.part + (size_t)2 * G * 2 * b;
-    a.counter = counter;
+    a.gate = cq ? static_cast<const int *>(scratch) : nullptr;
+    a.counter = counter + 1;
     a.err = err;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, counter, 1);
-    CUEV_LAUNCH_CHECK();
     hipLaunchKernelGGL(panel_qr_kernel<T>, dim3(G), dim3(kPQThreads), lds, s, a);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
