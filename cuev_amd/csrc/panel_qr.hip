@@ -339,7 +339,12 @@ template <typename T> struct CQArgs {
     int *err;
 };
 
-// out = P^T P over the first nr rows (P column-major, ld CHp); thread = 4x4 block.
+// Packed lower-triangle index of (i, j), i >= j, of a B x B symmetric matrix.
+template <int B> __device__ __forceinline__ int cq_lo(int i, int j) { return i + j * B - (j * (j + 1)) / 2; }
+
+// out (packed lower) = P^T P over the first nr rows (P column-major, ld CHp).
+// Thread = strided 4x4 block, rows ti + NB u and columns tj + NB v, so the 16
+// lanes with different ti read 16 different banks.
 template <int B>
 __device__ __forceinline__ void cq_gram(const double *P, int CHp, int nr, double *out, int tid) {
     constexpr int NB = B / 4;
@@ -350,13 +355,12 @@ __device__ __forceinline__ void cq_gram(const double *P, int CHp, int nr, double
     for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-    const double *pa = P + (size_t)(4 * ti) * CHp, *pb = P + (size_t)(4 * tj) * CHp;
     for (int r = 0; r < nr; ++r) {
         double x[4], y[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            x[u] = pa[u * CHp + r];
-            y[u] = pb[u * CHp + r];
+            x[u] = P[(size_t)(ti + NB * u) * CHp + r];
+            y[u] = P[(size_t)(tj + NB * u) * CHp + r];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -366,12 +370,40 @@ __device__ __forceinline__ void cq_gram(const double *P, int CHp, int nr, double
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) out[(4 * ti + u) + (size_t)(4 * tj + v) * B] = acc[u][v];
+        for (int v = 0; v < 4; ++v) {
+            const int i = ti + NB * u, j = tj + NB * v;
+            if (i >= j) out[cq_lo<B>(i, j)] = acc[u][v];
+        }
 }
 
-// P[rows r0..r1) <- P U for an upper-triangular U (LDS, column-major, ld LD), in
+// acc = X Y for B x B LDS matrices (column-major, ld B + 1; XT: X transposed),
+// thread = strided 4x4 block (ti + NB u, tj + NB v).
+template <int B, bool XT>
+__device__ __forceinline__ void cq_mm(double (&acc)[4][4], const double *X, const double *Y, int ti, int tj) {
+    constexpr int NB = B / 4, LD = B + 1;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int k = 0; k < B; ++k) {
+        double x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = ti + NB * u;
+            x[u] = XT ? X[k + i * LD] : X[i + k * LD];
+            y[u] = Y[k + (tj + NB * u) * LD];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * y[v];
+    }
+}
+
+// P[rows r0..r1) <- P U for an upper-triangular U (LDS, column-major, ld B + 1), in
 // place: rounds of whole 4-row blocks, all column blocks of a row block in the
-// same round (a row's new values depend on that row only).
+// same round (a row's new values depend on that row only). Columns are strided
+// over threads (tj + NB v) so the U reads of a wave spread over the banks.
 template <int B>
 __device__ __forceinline__ void cq_trmm(double *P, int CHp, int r0, int r1, const double *U, int tid) {
     constexpr int NB = B / 4, LD = B + 1;
@@ -387,12 +419,13 @@ __device__ __forceinline__ void cq_trmm(double *P, int CHp, int r0, int r1, cons
             for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
         const int rr = r0 + 4 * rb;
         if (act) {
-            for (int i = 0; i < 4 * tj + 4; ++i) {
+            const int imax = tj + NB * 3; // U(i, j) = 0 for i > j
+            for (int i = 0; i <= imax; ++i) {
                 double x[4], w[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) x[u] = P[(size_t)i * CHp + rr + u];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) w[v] = U[i + (4 * tj + v) * LD];
+                for (int v = 0; v < 4; ++v) w[v] = U[i + (tj + NB * v) * LD];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -405,18 +438,21 @@ __device__ __forceinline__ void cq_trmm(double *P, int CHp, int r0, int r1, cons
             for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    if (rr + u < r1) P[(size_t)(4 * tj + v) * CHp + rr + u] = acc[u][v];
+                    if (rr + u < r1) P[(size_t)(tj + NB * v) * CHp + rr + u] = acc[u][v];
         __syncthreads();
     }
 }
 
 template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel_cholqr_kernel(CQArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int NB = B / 4, LD = B + 1, BB = B * B;
+    constexpr int NB = B / 4, LD = B + 1, BB = B * B, BL = B * (B + 1) / 2;
     const int CH = a.CH, CHp = CH + 1;
     const int tid = threadIdx.x, g = blockIdx.x;
     const bool own = tid < NB * NB;
+    // contiguous 4x4 blocks (4 ti + u, 4 tj + v) for the factorizations, strided
+    // (si + NB u, sj + NB v) for the products
     const int ti = own ? tid % NB : 0, tj = own ? tid / NB : 0;
+    const int si = ti, sj = tj;
     double *P = reinterpret_cast<double *>(smem_raw); // [B][CHp]: panel(r, c) at P[c*CHp + r]
     double *M1 = P + (size_t)B * CHp;                 // [B][LD] column-major
     double *M2 = M1 + B * LD;
@@ -440,11 +476,11 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
     unsigned nbar = 0;
     for (int pass = 0; pass < 2; ++pass) {
         // ---- Gram matrix of the panel: partials, then fixed-order slice sums
-        cq_gram<B>(P, CHp, chl, a.part + (size_t)g * BB, tid);
+        cq_gram<B>(P, CHp, chl, a.part + (size_t)g * BL, tid);
         if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
         {
-            const int S = (BB + a.G - 1) / a.G;
-            const int e0 = g * S, ne = max(0, min(BB, e0 + S) - e0);
+            const int S = (BL + a.G - 1) / a.G;
+            const int e0 = g * S, ne = max(0, min(BL, e0 + S) - e0);
             int tpe = 1; // threads per entry (power of two)
             while (tpe < 64 && tpe * 2 * max(ne, 1) <= kCQThreads) tpe *= 2;
             const int per_round = kCQThreads / tpe;
@@ -458,7 +494,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
 #pragma unroll
                         for (int u = 0; u < 16; ++u) {
                             const int gg = g0 + u * tpe;
-                            v[u] = (gg < a.G) ? src[(size_t)gg * BB] : 0.0;
+                            v[u] = (gg < a.G) ? src[(size_t)gg * BL] : 0.0;
                         }
 #pragma unroll
                         for (int u = 0; u < 16; ++u) s += v[u];
@@ -476,17 +512,86 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
         }
         if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
 
-        // ---- G = L' D L'^T with X = L'^-1 (registers), R_p = D^1/2 L'^T
+        // ---- second pass, G = I + E with E tiny (the usual case): R_2 = I + F,
+        // F = up(E - up(E)^T up(E)) with up(M) = strict upper of M + diag(M)/2,
+        // leaves R_2^T R_2 - G = O(|E|^3); R_2^-1 = I - F + F^2 + O(|F|^3)
+        if (pass == 1) {
+            double e[4][4];
+            double emax = 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = si + NB * u, j = sj + NB * v;
+                    const double gij = own ? a.gsum[(i >= j) ? cq_lo<B>(i, j) : cq_lo<B>(j, i)] : 0.0;
+                    e[u][v] = own ? gij - (i == j ? 1.0 : 0.0) : 0.0;
+                    emax = fmax(emax, fabs(e[u][v]));
+                    if (own) M2[i + j * LD] = (i < j) ? e[u][v] : (i == j ? 0.5 * e[u][v] : 0.0);
+                }
+            red[tid] = (emax == emax) ? emax : 1.0; // NaN -> not small
+            __syncthreads();
+            for (int s2 = kCQThreads / 2; s2 > 0; s2 >>= 1) {
+                if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
+                __syncthreads();
+            }
+            const bool small = red[0] <= 1e-6;
+            __syncthreads();
+            if (small) {
+                double c[4][4];
+                cq_mm<B, true>(c, M2, M2, si, sj); // up(E)^T up(E)
+                __syncthreads();
+                if (own)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int i = si + NB * u, j = sj + NB * v;
+                            const double f = e[u][v] - c[u][v];
+                            M2[i + j * LD] = (i < j) ? f : (i == j ? 0.5 * f : 0.0);
+                        }
+                __syncthreads();
+                double f2[4][4], fr[4][4];
+                cq_mm<B, false>(f2, M2, M2, si, sj); // F^2
+                cq_mm<B, false>(fr, M2, M1, si, sj); // F R_1
+                double fo[4][4], r1[4][4];
+                if (own)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int i = si + NB * u, j = sj + NB * v;
+                            fo[u][v] = M2[i + j * LD];
+                            r1[u][v] = M1[i + j * LD];
+                        }
+                __syncthreads();
+                if (own)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int i = si + NB * u, j = sj + NB * v;
+                            M2[i + j * LD] = (i == j ? 1.0 : 0.0) - fo[u][v] + f2[u][v]; // R_2^-1
+                            M1[i + j * LD] = r1[u][v] + fr[u][v];                         // R_2 R_1
+                        }
+                __syncthreads();
+                cq_trmm<B>(P, CHp, 0, CH, M2, tid);
+                continue;
+            }
+        }
+
+        // ---- G = L' D L'^T with X = L'^-1 (registers), R_p = D^1/2 L'^T. Each step
+        // applies masked multiplier vectors to all 16 entries of a thread's block.
         double gr[4][4], xr[4][4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                gr[u][v] = own ? a.gsum[(4 * ti + u) + (size_t)(4 * tj + v) * B] : 0.0;
-                xr[u][v] = (4 * ti + u == 4 * tj + v) ? 1.0 : 0.0;
+                const int i = 4 * ti + u, j = 4 * tj + v;
+                gr[u][v] = own ? a.gsum[(i >= j) ? cq_lo<B>(i, j) : cq_lo<B>(j, i)] : 0.0;
+                xr[u][v] = (i == j) ? 1.0 : 0.0;
             }
         double *Rout = (pass == 0) ? M1 : M3;
-        if (tid < B) gd[tid] = a.gsum[tid + (size_t)tid * B];
+        if (tid < B) gd[tid] = a.gsum[cq_lo<B>(tid, tid)];
         if (own && tj == 0)
 #pragma unroll
             for (int u = 0; u < 4; ++u) cb[4 * ti + u] = gr[u][0];
@@ -505,28 +610,31 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     ok = false;
                     break;
                 }
-                const double rk = 1.0 / dk, rs = 1.0 / sqrt(dk);
-                double ci[4], cj[4], xk[4];
+                const double rk = 1.0 / dk, sq = sqrt(dk), rs = 1.0 / sq;
+                double ci[4], cim[4], cjm[4], xk[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     ci[u] = cb[p * B + 4 * ti + u];
-                    cj[u] = cb[p * B + 4 * tj + u];
+                    cim[u] = (4 * ti + u > k) ? ci[u] : 0.0;
+                    const double cj = cb[p * B + 4 * tj + u];
+                    cjm[u] = (4 * tj + u > k) ? cj * rk : 0.0;
                     xk[u] = xb[p * B + 4 * tj + u];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < 4; ++u) {
+                    const double lx = cim[u] * rk;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int i = 4 * ti + u, j = 4 * tj + v;
-                        if (i > k && j > k) gr[u][v] -= (ci[u] * cj[v]) * rk;
-                        if (i > k && j <= k) xr[u][v] -= (ci[u] * rk) * xk[v];
+                        gr[u][v] -= cim[u] * cjm[v];
+                        xr[u][v] -= lx * xk[v];
                     }
+                }
                 if (tid == 0) dv[k] = dk;
                 if (own && tj == 0)
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int i = 4 * ti + u;
-                        Rout[k + i * LD] = (i > k) ? ci[u] * rs : (i == k ? sqrt(dk) : 0.0);
+                        Rout[k + i * LD] = (i > k) ? ci[u] * rs : (i == k ? sq : 0.0);
                     }
                 const int kn = k + 1, vn = (v0 + 1) & 3, bn = kn >> 2;
                 if (kn < B) {
@@ -556,29 +664,13 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
         if (pass == 1) {
             // R = R_2 R_1 into M1
             double acc[4][4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-            if (own)
-                for (int k = 0; k < B; ++k) {
-                    double x[4], w[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        x[u] = M3[(4 * ti + u) + k * LD];
-                        w[u] = M1[k + (4 * tj + u) * LD];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * w[v];
-                }
+            cq_mm<B, false>(acc, M3, M1, si, sj);
             __syncthreads();
             if (own)
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) M1[(4 * ti + u) + (4 * tj + v) * LD] = acc[u][v];
+                    for (int v = 0; v < 4; ++v) M1[(si + NB * u) + (sj + NB * v) * LD] = acc[u][v];
             __syncthreads();
         }
     }
@@ -611,20 +703,21 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 const double mkk = cb[p * B + k];
                 const double s = (mkk >= 0.0) ? -1.0 : 1.0;
                 const double piv = mkk - s, rp = 1.0 / piv;
-                double li[4], rj[4], xk[4];
+                double li[4], lim[4], rj[4], rjm[4], xk[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     li[u] = cb[p * B + 4 * ti + u] * rp;
+                    lim[u] = (4 * ti + u > k) ? li[u] : 0.0;
                     rj[u] = rb[p * B + 4 * tj + u];
-                    xk[u] = xb[p * B + 4 * tj + u];
+                    rjm[u] = (4 * tj + u > k) ? rj[u] : 0.0;
+                    xk[u] = xb[p * B + 4 * tj + u]; // X(k, j) = 0 for j > k
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int i = 4 * ti + u, j = 4 * tj + v;
-                        if (i > k && j > k) mr[u][v] -= li[u] * rj[v];
-                        if (i > k && j <= k) xr[u][v] -= li[u] * xk[v];
+                        mr[u][v] -= lim[u] * rjm[v];
+                        xr[u][v] -= lim[u] * xk[v];
                     }
                 if (tid == 0) {
                     sv[k] = s;
@@ -684,8 +777,8 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     double x[4], w[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        x[u] = Um[(4 * ti + u) + k * LD] * sk;
-                        w[u] = P[(size_t)k * CHp + 4 * tj + u];
+                        x[u] = Um[(si + NB * u) + k * LD] * sk;
+                        w[u] = P[(size_t)k * CHp + sj + NB * u];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
@@ -696,7 +789,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) M1[(4 * ti + u) + (4 * tj + v) * LD] = acc[u][v];
+                    for (int v = 0; v < 4; ++v) M1[(si + NB * u) + (sj + NB * v) * LD] = acc[u][v];
         }
         // U^-1 by back substitution, one row per step: rhs rows i < k -= U(i, k) V(k, :)
         double vr[4][4];
@@ -716,23 +809,22 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 const int k = 4 * kb + v0, p = k & 1;
                 if (k == 0) break;
                 __syncthreads();
-                double uk[4], vk[4];
+                double ukm[4], vk[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    uk[u] = Um[(4 * ti + u) + k * LD];
+                    ukm[u] = (4 * ti + u < k) ? Um[(4 * ti + u) + k * LD] : 0.0;
                     vk[u] = xb[p * B + 4 * tj + u];
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        if (4 * ti + u < k) vr[u][v] -= uk[u] * vk[v];
+                    for (int v = 0; v < 4; ++v) vr[u][v] -= ukm[u] * vk[v];
                 const int kp = k - 1, up = (v0 + 3) & 3, bp = kp >> 2;
                 if (own && ti == bp) {
-                    const double d = Um[kp + kp * LD];
+                    const double d = 1.0 / Um[kp + kp * LD];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        vr[up][v] /= d;
+                        vr[up][v] *= d;
                         xb[(kp & 1) * B + 4 * tj + v] = vr[up][v];
                     }
                 }
@@ -747,8 +839,8 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     const int i = 4 * ti + u, j = 4 * tj + v;
                     M3[i + j * LD] = vr[u][v];
                     a.bc[i + (size_t)j * B] = vr[u][v];
-                    a.bc[BB + i + (size_t)j * B] = M1[i + j * LD];
                 }
+        for (int idx = tid; idx < BB; idx += kCQThreads) a.bc[BB + idx] = M1[(idx % B) + (idx / B) * LD];
         // Y rows 0..b-1 into P (unit lower L)
         for (int idx = tid; idx < BB; idx += kCQThreads) {
             const int r = idx % B, j = idx / B;
@@ -787,12 +879,12 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
             for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-            for (int i = 0; i < 4 * cj + 4; ++i) {
+            for (int i = 0; i <= cj + NB * 3; ++i) {
                 double x[4], w[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) x[u] = P[(size_t)i * CHp + rr + u];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) w[v] = M1[i + (4 * cj + v) * LD];
+                for (int v = 0; v < 4; ++v) w[v] = M1[i + (cj + NB * v) * LD];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -803,7 +895,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 if (rr + u < chl)
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
-                        a.W[(i64)(r0g + rr + u) + (i64)(4 * cj + v) * a.ldw] = (T)acc[u][v];
+                        a.W[(i64)(r0g + rr + u) + (i64)(cj + NB * v) * a.ldw] = (T)acc[u][v];
         }
     }
     if (g == 0) {

@@ -59,6 +59,49 @@ def test_panel_qr(torch_dev, rows, b, prec, tol):
         assert np.linalg.norm(rec - A0) / np.linalg.norm(A0) < tol
 
 
+@pytest.mark.parametrize("kind", ["zero", "dependent", "rank1"])
+def test_panel_qr_rank_deficient(torch_dev, kind):
+    """Panels the Cholesky QR path declines (a pivot of the Gram matrix at or below
+    1e-10 of its diagonal): the Householder kernel factors them, with the oracle's
+    (LAPACK dgeqr2) reflectors, T and R."""
+    torch = torch_dev
+    rows, b = 3000, 64
+    rng = np.random.default_rng(4)
+    if kind == "zero":
+        A0 = np.zeros((rows, b))
+    elif kind == "dependent":
+        A0 = rng.standard_normal((rows, b))
+        A0[:, 17] = 2.0 * A0[:, 3] - A0[:, 40]
+    else:
+        A0 = np.outer(rng.standard_normal(rows), rng.standard_normal(b))
+    with cv.Handle(torch.float64, rows, b, b) as h:
+        dA = to_dev(A0, torch.float64)
+        dY = empty_dev(rows, b, torch.float64)
+        h.dbbr_panel_qr(dA, dY, rows, b)
+        torch.cuda.synchronize()
+        Af, Y = from_dev(dA), from_dev(dY)
+        Tptr, _ = h.buffer("TRI")
+        T = raw_dev(Tptr, b * b, torch.float64).reshape((b, b), order="F")
+    Ao, Yo, To, tauo = O.panel_qr(A0, b)
+    sc = max(1.0, np.abs(A0).max())
+    # columns past a dependency carry reflectors that rounding decides; the
+    # factorization itself must hold to working precision
+    R = np.zeros_like(A0)
+    R[:b] = np.triu(Af[:b])
+    rec = R - Y @ (T @ (Y.T @ R))
+    assert np.linalg.norm(rec - A0) <= 1e-12 * max(np.linalg.norm(A0), 1.0)
+    Qb = np.eye(rows)[:, :2 * b] - Y @ (T @ Y[:2 * b].T)
+    assert np.linalg.norm(Qb.T @ Qb - np.eye(2 * b)) < 1e-12
+    if kind == "zero":
+        assert np.array_equal(Y, Yo) and np.all(T == To) and np.all(Af == 0)
+    else:
+        # the columns before the first dependent one (40: a_40 = 2 a_3 - a_17;
+        # rank 1: column 1) match the oracle
+        c = 40 if kind == "dependent" else 1
+        assert np.abs(Y[:, :c] - Yo[:, :c]).max() < 1e-11
+        assert np.abs(np.triu(Af[:b, :c]) - np.triu(Ao[:b, :c])).max() < 1e-11 * sc
+
+
 def _reduce_gpu(torch, A0, b, k, dt):
     n = A0.shape[0]
     h = cv.Handle(dt, n, b, k)
