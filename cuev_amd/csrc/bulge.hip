@@ -20,11 +20,14 @@
 //    finished with (a ring of 2b column slots). The Householder vector is formed
 //    by wave 0 alone (L <= 64 = one wave64), with FP64 norms.
 //  * Dependency: hop (s, cq, r1) touches packed columns [cq, r1] only, and the
-//    previous sweep's hops touch columns [cq'', r1''] with cq'' increasing, so
-//    sweep s may start the hop once sweep s-1 has completed its hop with
-//    cq'' = r1 (progress flag prog[s-1] >= r1). That is two hops of lag instead
-//    of the reference's margin of 3b columns (bulge.cu:193-199), which shortens
-//    the ~ (lag * n) critical path.
+//    previous sweep's hops touch columns [cq'', r1''] with cq'' increasing. Of
+//    the hop of sweep s-1 with cq'' = r1, hop (s, cq) shares column r1 alone, and
+//    that column is final (beta e_0 below the pivot) as soon as its reflector is
+//    formed. So each hop writes its column cq back and publishes progress
+//    prog[s] = cq right after the Householder step; sweep s starts the hop once
+//    prog[s-1] >= r1, overlapping the rest of that hop of sweep s-1 (which then
+//    touches column r1 only in its own LDS). The reference waits for a margin of
+//    3b columns (bulge.cu:193-199); the critical path is ~ n times the lag.
 //  * Flags are agent-scope acquire/release atomics, every wait is bounded, and
 //    the grid never exceeds the number of co-resident workgroups, so the lowest
 //    unfinished sweep can always proceed.
@@ -164,6 +167,21 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             }
             __syncthreads();
 
+            // ---- column cq is now final (rows above r0 untouched by this sweep, beta e_0
+            // below): write it back and publish progress cq (release) before the rest of
+            // the hop. The next sweep needs only this column from this hop (its window
+            // ends at column cq), so it overlaps the rest of the hop; this hop reads and
+            // writes column cq only in LDS from here on.
+            {
+                const int o = r0 - cq;
+                T *dst = Bp + (i64)cq * ldb;
+                const T *srcw = win + (size_t)(cq & (NSLOT - 1)) * WLD;
+                for (int d = tid; d < ldb; d += kBCThreads)
+                    dst[d] = (d == o) ? (T)s_beta : ((d > o && d < o + L) ? T(0) : srcw[d]);
+            }
+            wg_publish_fence();
+            if (tid == 0) flag_store_release(prog + s, cq);
+
             // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow. Thread = (row t, k segment);
             // each thread walks its segment starting at a row-dependent offset, so the
             // lanes of a wave (consecutive rows) hit distinct LDS banks both for
@@ -244,16 +262,17 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
 
             // ---- columns this sweep never touches again go back to global memory:
             // [cq, next cq - 1] (the next hop starts at s+1 after hop 0, at cq+b after)
+            // (column cq itself went out with the progress flag; the next hop's flag
+            // releases these stores)
             const int next_cq = (h == 0) ? s + 1 : cq + b;
             const int keep_lo = min(next_cq, r1 + 1);
-            store_cols(cq, keep_lo - 1);
+            store_cols(cq + 1, keep_lo - 1);
             held_lo = keep_lo;
             T *ucol = U + (i64)(r0 - s - 1) + (i64)s * ldu;
             for (int t = tid; t < L; t += kBCThreads) ucol[t] = sw[t];
-
-            // ---- publish progress (release)
-            wg_publish_fence();
-            if (tid == 0) flag_store_release(prog + s, cq);
+            // the next hop's loads overwrite the ring slots just stored (sweep 0 has no
+            // wait phase, whose barrier would otherwise separate them)
+            __syncthreads();
         }
         // the rest of the last window
         store_cols(held_lo, held_hi);
@@ -290,8 +309,10 @@ template <typename T, int B> int launch_chase(Workspace<T> &ws, T *Bp) {
     CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc_chase_kernel<T, B>,
                                                           kBCThreads, lds));
     const int resident = std::max(1, per_cu * device_cu_count());
-    // at most ~n/(2b) sweeps can be active at once; more workgroups would only spin
-    const int useful = (int)ceil_div(n, 2 * B) + 2;
+    // a sweep of H hops starts three hops after the previous one, so at most
+    // ~H/3 <= n/(3b) sweeps are active at once; more workgroups would only spin on
+    // CUs that the concurrent back-transform GEMMs can use
+    const int useful = (int)ceil_div(n, 3 * B) + 8;
     const int grid = std::max(1, std::min({nsweeps, resident, useful}));
     // algorithmic work: ~n^2/(2b) hops, each reading and writing b columns of the
     // packed band (2 * 2b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops

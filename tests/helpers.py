@@ -44,6 +44,19 @@ def raw_dev(ptr, count, dtype):
     return out.cpu().numpy().astype(np.float64)
 
 
+def raw_put(ptr, A, dtype):
+    """Copy numpy array A (column-major order) into a raw device pointer (workspace buffer)."""
+    import ctypes
+    import torch
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    h = np.asfortranarray(np.asarray(A, dtype=npdt)).ravel(order="F")
+    torch.cuda.synchronize()
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rc = lib.hipMemcpy(ctypes.c_void_p(ptr), h.ctypes.data_as(ctypes.c_void_p), h.nbytes, 1)
+    assert rc == 0, f"hipMemcpy failed {rc}"
+
+
 def eig_metrics(A, lam, V, lam_ref):
     """BASELINE metrics: eigenvalue error / ||A||_2, residual ||AV - V diag(lam)||_F/(n ||A||_F),
     orthogonality ||V^T V - I||_F / n."""

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import cuev_amd as cv
-from helpers import to_dev, from_dev, raw_dev, empty_dev
+from helpers import to_dev, from_dev, raw_dev, raw_put, empty_dev
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -167,6 +167,31 @@ def test_bc_chase(torch_dev, n, b, k, prec):
     evt = np.linalg.eigvalsh(O.tridiag_dense(d, e))
     tol = 1e-9 if prec == "d" else 1e-3
     assert np.abs(ev - evt).max() / max(1e-300, ev[-1] - ev[0]) < tol
+
+
+@pytest.mark.parametrize("n,seed", [(8000, 3), (5000, 9)])
+def test_chase_large_spectrum(torch_dev, n, seed):
+    """Many sweeps in flight at once (about n/(3b) workgroups): the tridiagonal
+    (d, e) keeps the band matrix's spectrum. Reference eigenvalues: LAPACK's banded
+    symmetric solver (scipy.linalg.eigvals_banded) on the same band."""
+    import scipy.linalg as sl
+    torch = torch_dev
+    b = 64
+    rng = np.random.default_rng(seed)
+    band = rng.standard_normal((b + 1, n))  # lower band storage: band[d, j] = A[j + d, j]
+    for d in range(1, b + 1):
+        band[d, n - d:] = 0.0
+    Bp = np.zeros((2 * b, n))
+    Bp[: b + 1] = band
+    with cv.Handle(torch.float64, n, b, 512) as h:
+        raw_put(h.buffer("B")[0], Bp, torch.float64)
+        h.bc_chase()
+        torch.cuda.synchronize()
+        d = raw_dev(h.buffer("D")[0], n, torch.float64)
+        e = raw_dev(h.buffer("E")[0], n, torch.float64)
+    lam = sl.eigvalsh_tridiagonal(d, e[: n - 1])
+    ref = sl.eigvals_banded(band, lower=True)
+    assert np.max(np.abs(np.sort(lam) - np.sort(ref))) < 1e-11 * np.max(np.abs(ref))
 
 
 @pytest.mark.parametrize("n,b,k", [(256, 32, 128), (384, 32, 128), (777, 32, 128), (1024, 32, 128),
