@@ -26,7 +26,10 @@
 //    formed. So each hop writes its column cq back and publishes progress
 //    prog[s] = cq right after the Householder step; sweep s starts the hop once
 //    prog[s-1] >= r1, overlapping the rest of that hop of sweep s-1 (which then
-//    touches column r1 only in its own LDS). The reference waits for a margin of
+//    touches column r1 only in its own LDS). After the first hop of a sweep the
+//    reflector needs only column cq, already in LDS and final with respect to
+//    sweep s-1, so a hop runs: reflector, write-back + flag, wait for sweep s-1,
+//    load the b new columns, rank-2 update. The reference waits for a margin of
 //    3b columns (bulge.cu:193-199); the critical path is ~ n times the lag.
 //  * Flags are agent-scope acquire/release atomics, every wait is bounded, and
 //    the grid never exceeds the number of co-resident workgroups, so the lowest
@@ -119,35 +122,42 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             }
             const int hi = min(n - 1, r1 + b);  // last row touched
 
-            // ---- wait for sweep s-1 to be done with columns <= r1 (acquire)
-            if (s > 0) {
-                if (tid == 0) {
-                    long long it = 0;
-                    int go = 1;
-                    while (flag_load_relaxed(prog + (s - 1)) < r1) {
-                        __builtin_amdgcn_s_sleep(1);
-                        if (++it > kSpinLimit) {
-                            __hip_atomic_store(err, (int)DEV_WATCHDOG, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                            go = 0;
-                            break;
+            // ---- wait for sweep s-1 to be done with columns <= r1 (acquire), then bring
+            // the window [cq, r1] into LDS: columns already held stay, the rest come from
+            // global memory (b columns per hop after the first)
+            auto wait_and_load = [&]() -> bool {
+                if (s > 0) {
+                    if (tid == 0) {
+                        long long it = 0;
+                        int go = 1;
+                        while (flag_load_relaxed(prog + (s - 1)) < r1) {
+                            __builtin_amdgcn_s_sleep(1);
+                            if (++it > kSpinLimit) {
+                                __hip_atomic_store(err, (int)DEV_WATCHDOG, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                                go = 0;
+                                break;
+                            }
                         }
+                        fence_acquire_agent();
+                        s_go = go;
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    s_go = go;
+                    __syncthreads();
+                    if (!s_go) return false;
                 }
+                if (held_hi < held_lo) load_cols(cq, r1);
+                else if (r1 - held_hi == b) load_b_cols(held_hi + 1);
+                else load_cols(max(cq, held_hi + 1), r1);
+                held_lo = cq;
+                held_hi = r1;
                 __syncthreads();
-                if (!s_go) return;
-            }
-
-            // ---- bring the window [cq, r1] into LDS: columns already held stay,
-            // the rest come from global memory (b columns per hop after the first)
-            if (held_hi < held_lo) load_cols(cq, r1);
-            else if (r1 - held_hi == b) load_b_cols(held_hi + 1);
-            else load_cols(max(cq, held_hi + 1), r1);
-            held_lo = cq;
-            held_hi = r1;
-            __syncthreads();
+                return true;
+            };
+            // After the first hop, column cq is already in LDS and final with respect
+            // to sweep s-1 (whose hops since the last wait touch columns > cq), so the
+            // reflector and the progress flag go first and the wait and the load
+            // overlap the next sweep's start.
+            if (h == 0 && !wait_and_load()) return;
 
             // ---- Householder vector on x = A[r0:r1, cq] (wave 0)
             if (wave == 0) {
@@ -181,6 +191,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             }
             wg_publish_fence();
             if (tid == 0) flag_store_release(prog + s, cq);
+            if (h > 0 && !wait_and_load()) return;
 
             // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow. Thread = (row t, k segment);
             // each thread walks its segment starting at a row-dependent offset, so the
@@ -309,10 +320,10 @@ template <typename T, int B> int launch_chase(Workspace<T> &ws, T *Bp) {
     CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc_chase_kernel<T, B>,
                                                           kBCThreads, lds));
     const int resident = std::max(1, per_cu * device_cu_count());
-    // a sweep of H hops starts three hops after the previous one, so at most
-    // ~H/3 <= n/(3b) sweeps are active at once; more workgroups would only spin on
+    // a sweep of H hops starts about two hops after the previous one, so at most
+    // ~H/2 <= n/(2b) sweeps are active at once; more workgroups would only spin on
     // CUs that the concurrent back-transform GEMMs can use
-    const int useful = (int)ceil_div(n, 3 * B) + 8;
+    const int useful = (int)ceil_div(n, 2 * B) + 8;
     const int grid = std::max(1, std::min({nsweeps, resident, useful}));
     // algorithmic work: ~n^2/(2b) hops, each reading and writing b columns of the
     // packed band (2 * 2b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops

@@ -73,22 +73,40 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Acquire load / release store of a progress flag at agent (device) scope.
-__device__ __forceinline__ int flag_load_acquire(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Agent-scope release and acquire across the XCDs' separate L2 caches. The
+// release is an L2 write-back (buffer_wbl2 sc1) and the acquire an L2/L1
+// invalidate (buffer_inv sc1); both complete asynchronously and count in vmcnt.
+// An atomic with release order lowers to the write-back immediately followed by
+// the flag store, so the flag could become visible before the write-back has
+// finished; and other waves may load right after the acquiring thread passes the
+// workgroup barrier. So each is followed by an explicit vmcnt(0) wait: the flag
+// goes out only after the write-back, and the barrier opens only after the
+// invalidate.
+__device__ __forceinline__ void vm_drain() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): gfx9 encoding, vmcnt in [3:0] and [15:14]
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
+__device__ __forceinline__ void fence_release_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    vm_drain();
+}
+__device__ __forceinline__ void fence_acquire_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    vm_drain();
+}
+// Release store of a progress flag / release increment of a counter at agent scope.
 __device__ __forceinline__ void flag_store_release(int *p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    fence_release_agent();
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned counter_add_release(unsigned *p, unsigned v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    fence_release_agent();
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned counter_load_acquire(const unsigned *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Polling loads: relaxed, so a spin does not invalidate caches on every poll
-// (an agent-scope acquire load is a load followed by buffer_inv sc1 on gfx950).
-// The waiter issues one acquire fence after the condition holds.
+// Polling loads: relaxed, so a spin does not invalidate caches on every poll.
+// The waiter calls fence_acquire_agent() once after the condition holds.
 __device__ __forceinline__ int flag_load_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -100,17 +118,12 @@ __device__ __forceinline__ unsigned counter_load_relaxed(const unsigned *p) {
 // publish: every wave waits until its own global stores have been acknowledged
 // by the L2 (s_waitcnt vmcnt(0); a plain workgroup barrier waits only for LDS),
 // then the workgroup barrier; after it ONE thread performs the agent-scope
-// release (the flag or counter atomic with release order: one L2 write-back,
-// buffer_wbl2, that covers every wave's stores, then the flag). consume: after
-// one thread has observed the flag with a relaxed poll, it issues the acquire
-// fence (invalidates the CU's L1 and the non-coherent L2 lines) before the
-// barrier releases the others.
-__device__ __forceinline__ void wave_stores_done() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST); // no compiler motion of memory ops across
-    // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): gfx9 encoding, vmcnt in [3:0] and [15:14]
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
+// release (flag_store_release / counter_add_release: one L2 write-back that covers
+// every wave's stores, waited for, then the flag). consume: after one thread has
+// observed the flag with a relaxed poll, it calls fence_acquire_agent (invalidates
+// the CU's L1 and the non-coherent L2 lines, waited for) before the barrier
+// releases the others.
+__device__ __forceinline__ void wave_stores_done() { vm_drain(); }
 __device__ __forceinline__ void wg_publish_fence() {
     wave_stores_done();
     __syncthreads();

@@ -66,7 +66,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned *counter, unsigned target,
                 break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        fence_acquire_agent();
         s_ok = ok;
     }
     __syncthreads();
