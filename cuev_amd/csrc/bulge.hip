@@ -320,10 +320,12 @@ template <typename T, int B> int launch_chase(Workspace<T> &ws, T *Bp) {
     CUEV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc_chase_kernel<T, B>,
                                                           kBCThreads, lds));
     const int resident = std::max(1, per_cu * device_cu_count());
-    // a sweep of H hops starts about two hops after the previous one, so at most
-    // ~H/2 <= n/(2b) sweeps are active at once; more workgroups would only spin on
-    // CUs that the concurrent back-transform GEMMs can use
-    const int useful = (int)ceil_div(n, 2 * B) + 8;
+    // a sweep of H hops starts about two hops after the previous one, so up to
+    // ~H/2 <= n/(2b) sweeps could be active at once; the grid stays a little below
+    // that, leaving CUs to the back-transform GEMMs that run beside the chase
+    // (measured at n = 16384, b = 64: 112 workgroups 1.19 s per solve, 136: 1.215 s,
+    // 84: 1.215 s)
+    const int useful = (int)ceil_div((i64)n * 13, 32 * B) + 8;
     const int grid = std::max(1, std::min({nsweeps, resident, useful}));
     // algorithmic work: ~n^2/(2b) hops, each reading and writing b columns of the
     // packed band (2 * 2b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops
