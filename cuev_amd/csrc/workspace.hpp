@@ -2,11 +2,12 @@
 // include/cuda/handle.h:61-100 and src/cuda/handle.cu:16-94).
 //
 // One hipMalloc'd pool carved into the buffers below. Differences from the
-// reference's pool: 64-bit offsets throughout; no D&C device scratch (the
-// tridiagonal stage runs host dstedc); U uses a shifted layout and M is a plain
-// n x n buffer (the back-transform kernels predicate rows >= n instead of
-// relying on a padded leading dimension); separate scratch for split-K GEMM
-// partials and the panel-QR grid reduction.
+// reference's pool: 64-bit offsets throughout; the device D&C keeps its own
+// buffers (dc.hip), so the pool holds none; U uses a shifted layout and M is a
+// plain n x n buffer (the back-transform kernels predicate rows >= n instead of
+// relying on a padded leading dimension); bc_back's block-reflector operands are
+// pre-packed in register order (BtT); separate scratch for split-K GEMM
+// partials and the panel-QR grid reduction and gate word.
 #pragma once
 
 #include "common.hpp"

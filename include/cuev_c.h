@@ -75,7 +75,7 @@ typedef struct cuev_solve_timer {
     float bt_copyin_ms;  /* direct order: M <- Q_d upload; reordered order: Q_d upload */
     float bt_qb_ms;      /* bc_back: M <- Q_b M (reordered order: Q_b built on the identity) */
     float bt_qs_ms;      /* sbr_back: M <- Q_s M */
-    float bt_copyout_ms; /* reordered order: evec <- (Q_s Q_b) Q_d GEMM; direct: 0 */
+    float bt_copyout_ms; /* reordered order: evec <- (Q_b^T Q_s^T)^T Q_d GEMM; direct: 0 */
     /* additions */
     float refine_ms;     /* mixed form: Ogita-Aishima refinement sweeps */
     float overlap_ms;    /* GPU back-transform time hidden under the host D&C */
@@ -97,7 +97,7 @@ typedef enum cuev_precision_t {
 typedef enum cuev_bt_mode_t {
     CUEV_BT_AUTO = 0,      /* reordered for n >= 2048, direct otherwise */
     CUEV_BT_DIRECT = 1,    /* evec <- Q_s (Q_b Q_d) after the D&C (reference order) */
-    CUEV_BT_REORDERED = 2  /* build Q_s Q_b on the GPU while the host D&C runs, then one GEMM */
+    CUEV_BT_REORDERED = 2  /* build Q_b^T Q_s^T on the GPU beside the chase and the D&C, then one GEMM */
 } cuev_bt_mode_t;
 
 typedef enum cuev_dc_mode_t {
@@ -171,7 +171,7 @@ typedef enum cuev_buffer_t {
     CUEV_BUF_E = 5,   /* n: tridiagonal off-diagonal (n-1 used) */
     CUEV_BUF_TRI = 6, /* b x b, ld b: last panel's block-reflector factor T */
     CUEV_BUF_TAU = 7, /* b: last panel's Householder scalars */
-    CUEV_BUF_M = 8    /* n x n, ld n: back-transform scratch (Q_s Q_b in the reordered order) */
+    CUEV_BUF_M = 8    /* n x n, ld n: back-transform scratch (Q_b^T Q_s^T in the reordered order) */
 } cuev_buffer_t;
 
 int cuev_handle_create(cuev_handle_t *handle, int dtype, int n, int nbw, int nk,
