@@ -152,21 +152,29 @@ __global__ __launch_bounds__(kBTThreads) void bt_pack_kernel(const T *__restrict
         }
 }
 
-// Position of a block in the application order (wave-uniform).
+// Position of a block in the application order (wave-uniform), with the block's
+// index in the packed operands, toff[g] + h.
 struct BtPos {
     int h0, g, h; // pass start hop, sweep group, hop
+    i64 id;
 };
 
+// Blocks (hops) of sweep group g: toff[g + 1] - toff[g].
+template <int B> __device__ __forceinline__ int bt_hops(int g, int n) { return (n - 2 - g * kBtGroup) / B + 1; }
+
 // Advance to the next block: hops of the group, then the group below, then the
-// next pass. Returns false after the last block.
-template <int B> __device__ __forceinline__ bool bt_next(BtPos &p, int n) {
+// next pass. Returns false after the last block. The index follows from the
+// previous one except at a pass start (toff read once per pass).
+template <int B> __device__ __forceinline__ bool bt_next(BtPos &p, int n, const i64 *__restrict__ toff) {
     constexpr int G = 128 / B;
     if (p.h + 1 < p.h0 + G && p.g * kBtGroup + 1 + (p.h + 1) * B <= n - 1) {
         ++p.h;
+        ++p.id;
         return true;
     }
     if (p.g > 0) {
         --p.g;
+        p.id += p.h0 - p.h - bt_hops<B>(p.g, n); // toff[g-1] = toff[g] - hops(g-1)
         p.h = p.h0;
         return true;
     }
@@ -175,6 +183,7 @@ template <int B> __device__ __forceinline__ bool bt_next(BtPos &p, int n) {
     if (smax < 0) return false;
     p.g = smax / kBtGroup;
     p.h = p.h0;
+    p.id = toff[p.g] + p.h;
     return true;
 }
 
@@ -184,17 +193,21 @@ template <int B> __device__ __forceinline__ bool bt_prev(BtPos &p, int n) {
     constexpr int G = 128 / B;
     if (p.h > p.h0) {
         --p.h;
+        --p.id;
         return true;
     }
     if (p.g < min(n - 3, n - 2 - p.h0 * B) / kBtGroup) {
+        const i64 t = p.id - p.h + bt_hops<B>(p.g, n); // toff[g+1] = toff[g] + hops(g)
         ++p.g;
         p.h = min(p.h0 + G - 1, (n - 2 - p.g * kBtGroup) / B);
+        p.id = t + p.h;
         return true;
     }
     if (p.h0 == 0) return false;
     p.h0 -= G;
     p.g = 0;
     p.h = min(p.h0 + G - 1, (n - 2) / B);
+    p.id = p.h; // toff[0] = 0
     return true;
 }
 
@@ -266,21 +279,25 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
 #define BT_STORE16(r0, b0)                                                                         \
     do {                                                                                           \
         const int r_ = (r0) + li;                                                                  \
-        if (r_ < n) {                                                                              \
-            const int o_ = wb + ((b0) + li) * LDW + q;                                             \
-            _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                     \
-                const int col_ = wcol + q + 4 * i_;                                                \
-                if (col_ < ncols) M[r_ + (i64)col_ * ldm] = win[o_ + 4 * i_];                      \
-            }                                                                                      \
+        const int o_ = wb + ((b0) + li) * LDW + q;                                                 \
+        T v_[4];                                                                                   \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) v_[i_] = win[o_ + 4 * i_];                \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                         \
+            const int col_ = wcol + q + 4 * i_;                                                    \
+            if (r_ < n && col_ < ncols) M[r_ + (i64)col_ * ldm] = v_[i_];                          \
         }                                                                                          \
     } while (0)
-// buffer rows [s0, s0+16) -> [d0, d0+16)
-#define BT_MOVE16(s0, d0)                                                                          \
+// buffer rows [s0, s0 + WS - 16) -> [d0, d0 + WS - 16): all read, then all written
+// (so the two ranges may overlap)
+#define BT_MOVE(s0, d0)                                                                            \
     do {                                                                                           \
-        T v_[4];                                                                                   \
+        constexpr int NM_ = (WS - 16) / 16;                                                        \
+        T v_[NM_][4];                                                                              \
         const int os_ = wb + ((s0) + li) * LDW + q, od_ = wb + ((d0) + li) * LDW + q;              \
-        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) v_[i_] = win[os_ + 4 * i_];               \
-        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[od_ + 4 * i_] = v_[i_];               \
+        _Pragma("unroll") for (int c_ = 0; c_ < NM_; ++c_)                                         \
+            _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) v_[c_][i_] = win[os_ + 16 * c_ * LDW + 4 * i_]; \
+        _Pragma("unroll") for (int c_ = 0; c_ < NM_; ++c_)                                         \
+            _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[od_ + 16 * c_ * LDW + 4 * i_] = v_[c_][i_]; \
     } while (0)
 
     BtPos p;
@@ -293,13 +310,14 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
         p.g = min(last, n - 2) / NG;
         p.h = 0;
     }
+    p.id = toff[p.g] + p.h;
     // the window moves down the buffer going up the groups (TR), up it going down
     int top = p.g * NG + 1 + p.h0 * B; // first window row
     int org = TR ? top : top - (RS - WS);
     BtPos pn = p;
-    bool more = TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n);
-    T cur[NOP], nxt[NOP];
-    bt_load_ops<T, NC>(cur, ops, toff[p.g] + p.h, lane);
+    bool more = TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n, toff);
+    T opa[NOP], opb[NOP];
+    bt_load_ops<T, NC>(opa, ops, p.id, lane);
     for (int c = 0; c < WS; c += 16) {
         T v[4];
         BT_LOAD16(top + c, v);
@@ -308,14 +326,19 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
     // this lane's first element of a block tile: row kidx(0, q), column li; the
     // others are at constant offsets (kidx(r, q) = kidx(0, q) + kidx(r, 0))
     const int lofs = wb + MB::kidx(0, q) * LDW + li;
+    // rows entering at the next group change (below the window going up the groups
+    // (TR), above it going down), loaded as soon as the window's position is known so
+    // that their latency is spread over the group's blocks
     T ent[4];
-    for (;;) {
-        // the next block's operands, in flight during this block's products
-        // (unconditional, a repeat of this block when nothing follows: no branch)
-        bt_load_ops<T, NC>(nxt, ops, more ? toff[pn.g] + pn.h : toff[p.g] + p.h, lane);
-        // rows entering with the next group (the same rows for every block of a group):
-        // above the window going down the groups, below it going up (TR)
-        BT_LOAD16(TR ? top + WS : top - 16, ent);
+    BT_LOAD16(TR ? top + WS : top - 16, ent);
+
+    // One block with operands cur; the next block's operands are loaded into nxt
+    // first (a repeat of this block when nothing follows: no branch). Returns false
+    // after the last block. Called alternately with the two register sets, so no
+    // operand is copied and the loads have a whole block to arrive.
+    auto step = [&](T(&cur)[NOP], T(&nxt)[NOP]) -> bool {
+        bt_load_ops<T, NC>(nxt, ops, more ? pn.id : p.id, lane);
+        __builtin_amdgcn_sched_barrier(0);
 
         // ---- block (g, h): rows [base, base + 16 NKB) of the window
         const int base = p.g * NG + 1 + p.h * B;
@@ -345,7 +368,7 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
         if (!more || pn.h0 != p.h0) {
             // pass done: write the window back
             for (int c = 0; c < WS; c += 16) BT_STORE16(top + c, top + c - org);
-            if (!more) break;
+            if (!more) return false;
             wave_stores_done(); // the next pass reads these rows back through other lanes
             top = pn.g * NG + 1 + pn.h0 * B;
             org = TR ? top : top - (RS - WS);
@@ -354,41 +377,45 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
                 BT_LOAD16(top + c, v);
                 BT_PUT16(top + c - org, v);
             }
+            BT_LOAD16(TR ? top + WS : top - 16, ent);
         } else if (pn.g != p.g) {
             if (TR) {
                 // group g -> g+1: 16 rows leave at the top, 16 enter at the bottom
                 BT_STORE16(top, top - org);
                 top += 16;
                 if (top + WS - org > RS) {
-                    // rows [top, top + WS - 16) back to the start of the buffer (moving
-                    // down by >= 16 rows: lowest first)
-                    for (int c = 0; c < WS - 16; c += 16) BT_MOVE16(top - org + c, c);
+                    // rows [top, top + WS - 16) back to the start of the buffer
+                    BT_MOVE(top - org, 0);
                     org = top;
                 }
                 BT_PUT16(top + WS - 16 - org, ent);
+                BT_LOAD16(top + WS, ent);
             } else {
                 // group g -> g-1: 16 rows leave at the bottom, 16 enter at the top
                 BT_STORE16(top + WS - 16, top + WS - 16 - org);
                 top -= 16;
                 if (top < org) {
-                    // rows [top + 16, top + WS) to the end of the buffer (moving up by
-                    // >= 16 rows: highest first)
+                    // rows [top + 16, top + WS) to the end of the buffer
                     const int norg = top - (RS - WS);
-                    for (int c = WS - 32; c >= 0; c -= 16) BT_MOVE16(top + 16 - org + c, top + 16 - norg + c);
+                    BT_MOVE(top + 16 - org, top + 16 - norg);
                     org = norg;
                 }
                 BT_PUT16(top - org, ent);
+                BT_LOAD16(top - 16, ent);
             }
         }
         p = pn;
-        more = more && (TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n));
-#pragma unroll
-        for (int c = 0; c < NOP; ++c) cur[c] = nxt[c];
+        more = more && (TR ? bt_prev<B>(pn, n) : bt_next<B>(pn, n, toff));
+        return true;
+    };
+    for (;;) {
+        if (!step(opa, opb)) break;
+        if (!step(opb, opa)) break;
     }
 #undef BT_LOAD16
 #undef BT_PUT16
 #undef BT_STORE16
-#undef BT_MOVE16
+#undef BT_MOVE
 }
 
 template <typename T, int B>
