@@ -232,9 +232,10 @@ __device__ __forceinline__ void bt_load_ops(T (&o)[2 * NP], const T *__restrict_
 // 16 out); when it reaches the end of the buffer, its rows move back to the other
 // end in one copy (every 4 groups), so a block's 16 NKB rows are always contiguous
 // and every LDS access of a block is one base address plus a constant. The next
-// block's operands and the next group's entering rows are loaded into registers
-// while the current block computes. Rows past n are read from zp (zeros) and not
-// written.
+// block's operands are loaded into the other of two register sets at the start of
+// each block, and the rows entering at a group change are loaded at the previous
+// change, so neither load is waited for inside the block that issues it. Rows past
+// n are read from zp (zeros) and not written.
 template <typename T, int B, bool TR>
 __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict__ ops,
                                                              const i64 *__restrict__ toff,
