@@ -128,7 +128,11 @@ __global__ __launch_bounds__(kBTThreads) void bt_pack_kernel(const T *__restrict
         for (int i = j + 1; i < NG; ++i) st[wave][i][j] = T(0);
     }
     __syncthreads();
-    if (!act) return;
+    if (!act) {
+        if (id == nblocks) // an all-zero block after the last (the identity, for the paired kernel)
+            for (int c = 0; c < NOP; ++c) ops[id * (i64)(NOP * 64) + c * 64 + lane] = T(0);
+        return;
+    }
     const int q = lane >> 4, li = lane & 15;
     T *ob = ops + id * (i64)(NOP * 64) + 2 * lane;
 #pragma unroll
@@ -419,13 +423,273 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_kernel(const T *__restrict
 #undef BT_MOVE
 }
 
+// Passes of two hops for the paired kernel below. The blocks of a pass, in the
+// order of bc_back_kernel (groups down, hops up), taken two at a time as steps
+// {(g, h0+1), (g-1, h0)}: the two blocks of a step cover disjoint rows (the first
+// ends where the second starts), so they are independent and the step is the same
+// sequence of blocks. In the order of Q_b^T M (TR: groups up, hops down) the steps
+// are {(g, h0), (g+1, h0+1)}. Either way one step covers the rows of a window of
+// 2 NKB 16-row tiles: its lower block the first NKB, its upper block the others.
+struct BtStep {
+    int h0, g;  // pass start hop, step group (see bt_step_blocks)
+    int gA, gB; // last group with hop h0, with hop h0 + 1 (-1: none)
+    i64 tg;     // forward: toff[g]; TR: toff[g + 1]
+};
+
+template <int B> __device__ __forceinline__ int bt_gmax(int n, int h) {
+    const int m = min(n - 3, n - 2 - h * B);
+    return m < 0 ? -1 : m / kBtGroup;
+}
+
+template <int B, bool TR> __device__ __forceinline__ void bt_step_first(BtStep &s, int n, const i64 *__restrict__ toff) {
+    s.h0 = TR ? ((n - 2) / B) / 2 * 2 : 0;
+    s.gA = bt_gmax<B>(n, s.h0);
+    s.gB = bt_gmax<B>(n, s.h0 + 1);
+    s.g = TR ? (s.gB >= 0 ? -1 : 0) : s.gA + 1;
+    s.tg = toff[TR ? s.g + 1 : s.g];
+}
+
+// Next step; false after the last. The offsets follow from the hop counts except
+// at a pass start.
+template <int B, bool TR> __device__ __forceinline__ bool bt_step_next(BtStep &s, int n, const i64 *__restrict__ toff) {
+    if (TR) {
+        if (s.g < s.gA) {
+            ++s.g;
+            s.tg += bt_hops<B>(s.g, n);
+            return true;
+        }
+        s.h0 -= 2;
+        if (s.h0 < 0) return false;
+        s.gA = bt_gmax<B>(n, s.h0);
+        s.gB = bt_gmax<B>(n, s.h0 + 1);
+        s.g = s.gB >= 0 ? -1 : 0;
+        s.tg = toff[s.g + 1];
+        return true;
+    }
+    if (s.g > (s.gB >= 0 ? 0 : 1)) {
+        s.tg -= bt_hops<B>(s.g - 1, n);
+        --s.g;
+        return true;
+    }
+    s.h0 += 2;
+    s.gA = bt_gmax<B>(n, s.h0);
+    if (s.gA < 0) return false;
+    s.gB = bt_gmax<B>(n, s.h0 + 1);
+    s.g = s.gA + 1;
+    s.tg = toff[s.g];
+    return true;
+}
+
+// The two blocks of a step: packed index (il, iu; an absent block, at a pass end,
+// gets the all-zero block nblocks, which leaves its rows unchanged) and the
+// window's first row.
+template <int B, bool TR>
+__device__ __forceinline__ void bt_step_blocks(const BtStep &s, int n, i64 nblocks, i64 &il, i64 &iu, int &top) {
+    if (TR) { // lower (g, h0), upper (g+1, h0+1)
+        il = s.g >= 0 ? s.tg - bt_hops<B>(s.g, n) + s.h0 : nblocks;
+        iu = s.g + 1 <= s.gB ? s.tg + s.h0 + 1 : nblocks;
+        top = s.g * kBtGroup + 1 + s.h0 * B;
+    } else { // lower (g-1, h0), upper (g, h0+1)
+        il = s.g >= 1 ? s.tg - bt_hops<B>(s.g - 1, n) + s.h0 : nblocks;
+        iu = s.g <= s.gB ? s.tg + s.h0 + 1 : nblocks;
+        top = (s.g - 1) * kBtGroup + 1 + s.h0 * B;
+    }
+}
+
+// One half of a block's packed operands: V (half 0) or W (half 1), NP pairs.
+template <typename T, int NP>
+__device__ __forceinline__ void bt_load_half(T (&o)[2 * NP], const T *__restrict__ ops, i64 id, int half, int lane) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const T2 *p = reinterpret_cast<const T2 *>(ops + id * (i64)(4 * NP * 64)) + (half * NP) * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < NP; ++c) {
+        const T2 v = p[c * 64];
+        o[2 * c] = v.x;
+        o[2 * c + 1] = v.y;
+    }
+}
+
+// M <- Q_b M (TR: Q_b^T M) for columns [0, ncols), one wave per 16 columns, two
+// blocks per step (see BtStep): the two blocks' products are interleaved, so the
+// MFMA pipe has independent work while either block's chain drains, and a step's
+// fixed costs (LDS round trip, window bookkeeping) are shared by two blocks. The
+// window lives in a ring of 2 NKB + 1 tiles of 16 rows per wave (24 KB at b = 64,
+// so the kernel leaves LDS for the tridiagonal solver's GEMMs running beside it);
+// a step slides it by one tile, storing the tile that leaves and putting the one
+// that enters (loaded a step earlier) into the free slot. The next step's operands
+// are loaded into the registers of the current ones as these die: V after the X
+// products, W after the update.
+template <typename T, int B, bool TR>
+__global__ __launch_bounds__(kBBThreads) void bc_back_pair_kernel(const T *__restrict__ ops,
+                                                                  const i64 *__restrict__ toff,
+                                                                  i64 nblocks, const T *__restrict__ zp,
+                                                                  T *M, i64 ldm, int n, int ncols) {
+    using MB = MfmaBt<T>;
+    using V4 = typename MB::v4;
+    constexpr int NKB = B / 16 + 1; // tiles of a block
+    constexpr int NC = 4 * NKB;     // 4-row K chunks of a block; values per operand half
+    constexpr int NW = 2 * NKB;     // window tiles
+    constexpr int NS = NW + 1;      // ring slots: the window and the entering tile
+    constexpr int LDW = 17;
+    constexpr int SL = 16 * LDW; // one slot
+    __shared__ T win[(kBBThreads / 64) * NS * SL];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, q = lane >> 4, li = lane & 15;
+    const int wcol = (blockIdx.x * (kBBThreads / 64) + wave) * 16;
+    if (wcol >= ncols) return; // whole wave
+    const int wb = wave * (NS * SL);
+    const int lofs = wb + MB::kidx(0, q) * LDW + li; // a block tile's element of this lane
+    int tb = 0;                                      // ring slot of window tile 0
+    auto slot = [&](int t) { // ring slot of window tile t in [0, NS)
+        const int s = tb + t;
+        return s >= NS ? s - NS : s;
+    };
+
+// matrix rows [r0, r0+16) of the wave's 16 columns: lane = (row li, columns q + 4i)
+#define BP_LOAD16(r0, v)                                                                           \
+    do {                                                                                           \
+        const int r_ = (r0) + li;                                                                  \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                         \
+            const int col_ = wcol + q + 4 * i_;                                                    \
+            const bool ok_ = r_ >= 0 && r_ < n && col_ < ncols;                                    \
+            (v)[i_] = *(ok_ ? M + r_ + (i64)col_ * ldm : zp);                                      \
+        }                                                                                          \
+    } while (0)
+#define BP_PUT16(s, v)                                                                             \
+    do {                                                                                           \
+        const int o_ = wb + (s) * SL + li * LDW + q;                                               \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) win[o_ + 4 * i_] = (v)[i_];               \
+    } while (0)
+#define BP_STORE16(s, r0)                                                                          \
+    do {                                                                                           \
+        const int r_ = (r0) + li;                                                                  \
+        const int o_ = wb + (s) * SL + li * LDW + q;                                               \
+        T v_[4];                                                                                   \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) v_[i_] = win[o_ + 4 * i_];                \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                         \
+            const int col_ = wcol + q + 4 * i_;                                                    \
+            if (r_ >= 0 && r_ < n && col_ < ncols) M[r_ + (i64)col_ * ldm] = v_[i_];               \
+        }                                                                                          \
+    } while (0)
+
+    BtStep cur;
+    bt_step_first<B, TR>(cur, n, toff);
+    int top;
+    i64 il, iu;
+    bt_step_blocks<B, TR>(cur, n, nblocks, il, iu, top);
+    T lv[NC], lw[NC], uv[NC], uw[NC];
+    bt_load_half<T, NC / 2>(lv, ops, il, 0, lane);
+    bt_load_half<T, NC / 2>(uv, ops, iu, 0, lane);
+    bt_load_half<T, NC / 2>(lw, ops, il, 1, lane);
+    bt_load_half<T, NC / 2>(uw, ops, iu, 1, lane);
+    for (int t = 0; t < NW; ++t) {
+        T v[4];
+        BP_LOAD16(top + 16 * t, v);
+        BP_PUT16(t, v);
+    }
+    T ent[4];
+    bool putent = false;
+    for (;;) {
+        if (putent) BP_PUT16(slot(TR ? NW - 1 : 0), ent);
+        BtStep nx = cur;
+        const bool more = bt_step_next<B, TR>(nx, n, toff);
+        int ntop = top;
+        i64 nil = il, niu = iu;
+        if (more) bt_step_blocks<B, TR>(nx, n, nblocks, nil, niu, ntop);
+        // the tile entering at the next step, if it is in this pass
+        BP_LOAD16(TR ? top + 16 * NW : top - 16, ent);
+
+        // ---- the two blocks: lower on window tiles [0, NKB), upper on [NKB, NW)
+        T ml[NC], mu[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int r = MB::kidx(c & 3, 0) * LDW;
+            ml[c] = win[lofs + slot(c >> 2) * SL + r];
+            mu[c] = win[lofs + slot(NKB + (c >> 2)) * SL + r];
+        }
+        // X = V^T M_J, two chains per block
+        V4 xl = {T(0), T(0), T(0), T(0)}, xl1 = xl, xu = xl, xu1 = xl;
+#pragma unroll
+        for (int c = 0; c < NC; c += 2) {
+            xl = MB::mma(lv[c], ml[c], xl);
+            xu = MB::mma(uv[c], mu[c], xu);
+            xl1 = MB::mma(lv[c + 1], ml[c + 1], xl1);
+            xu1 = MB::mma(uv[c + 1], mu[c + 1], xu1);
+        }
+        xl += xl1;
+        xu += xu1;
+        // the next step's V (a repeat of this step's when nothing follows)
+        bt_load_half<T, NC / 2>(lv, ops, nil, 0, lane);
+        bt_load_half<T, NC / 2>(uv, ops, niu, 0, lane);
+        // M_J += W X (W = -V T, or -V T^T). All products are issued before any
+        // store: a store waits for its product, and in-order issue would hold the
+        // remaining products behind it.
+        V4 al[NKB], au[NKB];
+#pragma unroll
+        for (int t = 0; t < NKB; ++t) {
+            al[t] = V4{ml[4 * t], ml[4 * t + 1], ml[4 * t + 2], ml[4 * t + 3]};
+            au[t] = V4{mu[4 * t], mu[4 * t + 1], mu[4 * t + 2], mu[4 * t + 3]};
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int t = 0; t < NKB; ++t) {
+                al[t] = MB::mma(lw[4 * t + r], xl[r], al[t]);
+                au[t] = MB::mma(uw[4 * t + r], xu[r], au[t]);
+            }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < NKB; ++t) {
+            const int ol = lofs + slot(t) * SL, ou = lofs + slot(NKB + t) * SL;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                win[ol + MB::kidx(r, 0) * LDW] = al[t][r];
+                win[ou + MB::kidx(r, 0) * LDW] = au[t][r];
+            }
+        }
+
+        // ---- window
+        if (!more || nx.h0 != cur.h0) {
+            for (int t = 0; t < NW; ++t) BP_STORE16(slot(t), top + 16 * t);
+            if (!more) break;
+            wave_stores_done(); // the next pass reads these rows back through other lanes
+            tb = 0;
+            for (int t = 0; t < NW; ++t) {
+                T v[4];
+                BP_LOAD16(ntop + 16 * t, v);
+                BP_PUT16(t, v);
+            }
+            putent = false;
+        } else {
+            if (TR) {
+                BP_STORE16(tb, top);
+                tb = tb + 1 == NS ? 0 : tb + 1;
+            } else {
+                BP_STORE16(slot(NW - 1), top + 16 * (NW - 1));
+                tb = tb == 0 ? NS - 1 : tb - 1;
+            }
+            putent = true;
+        }
+        // the next step's W
+        bt_load_half<T, NC / 2>(lw, ops, nil, 1, lane);
+        bt_load_half<T, NC / 2>(uw, ops, niu, 1, lane);
+        cur = nx;
+        il = nil;
+        iu = niu;
+        top = ntop;
+    }
+#undef BP_LOAD16
+#undef BP_PUT16
+#undef BP_STORE16
+}
+
 template <typename T, int B>
 int launch_bc_back(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, bool tr) {
     const int n = ws.n;
     if (ws.bt_blocks <= 0) return CUEV_STATUS_SUCCESS;
     {
         prof::Scope sc(s, "bt.bc_pack", (double)ws.bt_blocks * kBtGroup * kBtGroup * B);
-        const dim3 grid((unsigned)ceil_div(ws.bt_blocks, 4));
+        const dim3 grid((unsigned)ceil_div(ws.bt_blocks + 1, 4)); // + the zero block
         if (tr)
             hipLaunchKernelGGL((bt_pack_kernel<T, B, true>), grid, dim3(kBTThreads), 0, s, U, ws.ldu,
                                ws.bt_off, ws.bt_groups, ws.bt_blocks, n, ws.BtT);
@@ -438,12 +702,22 @@ int launch_bc_back(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, b
     prof::Scope sc(s, "bt.bc_back", 2.0 * (double)n * n * n);
     const i64 blocks = ceil_div(n, 16 * (kBBThreads / 64));
     const T *zp = U + ws.ldu * (i64)n; // zeros past U's last column
-    if (tr)
-        hipLaunchKernelGGL((bc_back_kernel<T, B, true>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
-                           ws.BtT, ws.bt_off, zp, M, ldm, n, n);
-    else
-        hipLaunchKernelGGL((bc_back_kernel<T, B, false>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
-                           ws.BtT, ws.bt_off, zp, M, ldm, n, n);
+    if constexpr (B == 64) {
+        // two blocks per step (passes of two hops: the default band's window fits)
+        if (tr)
+            hipLaunchKernelGGL((bc_back_pair_kernel<T, B, true>), dim3((unsigned)blocks), dim3(kBBThreads), 0,
+                               s, ws.BtT, ws.bt_off, ws.bt_blocks, zp, M, ldm, n, n);
+        else
+            hipLaunchKernelGGL((bc_back_pair_kernel<T, B, false>), dim3((unsigned)blocks), dim3(kBBThreads), 0,
+                               s, ws.BtT, ws.bt_off, ws.bt_blocks, zp, M, ldm, n, n);
+    } else {
+        if (tr)
+            hipLaunchKernelGGL((bc_back_kernel<T, B, true>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
+                               ws.BtT, ws.bt_off, zp, M, ldm, n, n);
+        else
+            hipLaunchKernelGGL((bc_back_kernel<T, B, false>), dim3((unsigned)blocks), dim3(kBBThreads), 0, s,
+                               ws.BtT, ws.bt_off, zp, M, ldm, n, n);
+    }
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
 }

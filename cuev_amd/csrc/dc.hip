@@ -581,7 +581,7 @@ template <typename T> void dc_state_destroy(DcState<T> *st) {
 
 template <typename T>
 int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q, i64 ldq, T *eval,
-             T *Qc, T *S, T *Qn, Scratch gs, int leaf) {
+             T *Qc, T *S, T *Qn, Scratch gs, int leaf, hipEvent_t gate, int gate_from) {
     if (n <= 0) return CUEV_STATUS_SUCCESS;
     if (n > st.n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "dc: state sized for %d, got n=%d", st.n, n);
     if (ldq < n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "dc: ldq < n");
@@ -687,6 +687,12 @@ int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q,
             md0[i].N = lev[i]->N;
             md0[i].n1 = lev[i]->n1;
             maxN = std::max(maxN, lev[i]->N);
+        }
+        if (gate && maxN >= gate_from) {
+            // from here on the merges have the GPU to themselves: full-size tiles
+            CUEV_HIP(hipStreamWaitEvent(s, gate, 0));
+            gate = nullptr;
+            gs.small_tiles = 0;
         }
         // z and the current eigenvalues to the host
         CUEV_HIP(hipMemcpyAsync(st.d_up + L.md0, md0, sizeof(MergeDesc) * nm, hipMemcpyHostToDevice, s));
@@ -842,8 +848,9 @@ template int dc_state_create<double>(DcState<double> *&, int);
 template void dc_state_destroy<float>(DcState<float> *);
 template void dc_state_destroy<double>(DcState<double> *);
 template int dc_solve<float>(DcState<float> &, hipStream_t, int, const float *, const float *, float *,
-                             i64, float *, float *, float *, float *, Scratch, int);
+                             i64, float *, float *, float *, float *, Scratch, int, hipEvent_t, int);
 template int dc_solve<double>(DcState<double> &, hipStream_t, int, const double *, const double *,
-                              double *, i64, double *, double *, double *, double *, Scratch, int);
+                              double *, i64, double *, double *, double *, double *, Scratch, int,
+                              hipEvent_t, int);
 
 } // namespace cuev

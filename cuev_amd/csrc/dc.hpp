@@ -25,9 +25,13 @@ template <typename T> void dc_state_destroy(DcState<T> *st);
 // Qc, S and Qn are device work buffers of n*n elements each; they may alias
 // nothing else used by the call. All work goes on stream s, which the call
 // synchronises before returning. gs is split-K scratch owned by s (may be empty).
-// leaf <= 0 selects the default leaf size.
+// leaf <= 0 selects the default leaf size. gate (may be null) is an event the
+// stream waits for before the first tree level whose largest merge has gate_from
+// rows or more: a caller running other GPU-wide work beside the D&C lets the
+// small, latency-bound merges overlap it and keeps the large merges, whose GEMMs
+// want the whole GPU, until it is done.
 template <typename T>
 int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q, i64 ldq, T *eval,
-             T *Qc, T *S, T *Qn, Scratch gs, int leaf);
+             T *Qc, T *S, T *Qn, Scratch gs, int leaf, hipEvent_t gate = nullptr, int gate_from = 0);
 
 } // namespace cuev

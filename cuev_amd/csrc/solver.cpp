@@ -48,6 +48,10 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
     return ms;
 }
+// Smallest merge of the device D&C that waits for the back-transform running beside
+// it: the three top tree levels, which hold nearly all of the D&C's GEMM work
+// (measured at n = 16384 against other levels, no gate, and no overlap at all).
+int kDcGateFrom(int n) { return n / 4; }
 } // namespace
 
 template <typename T>
@@ -150,11 +154,14 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
         // work buffers. Reordered: the D&C runs on its own stream beside the
         // back-transform still queued on s, without split-K (its scratch is s's).
         if (reordered) {
-            // small GEMM tiles: the merges then fit beside the back-transform's waves
+            // small GEMM tiles: the merges then fit beside the back-transform's
+            // waves; the top tree levels (nearly all of the D&C's GEMM work) wait
+            // for the back-transform, so the two GPU-wide phases do not share the
+            // GPU and only the small, latency-bound merges overlap it
             Scratch dcs;
             dcs.small_tiles = 1;
             CUEV_TRY(dc_solve<T>(*ctx.dc, ctx.dc_stream, n, ctx.h_d, ctx.h_e, ctx.dcQ, n, eval, A, evec,
-                                 ctx.dcQn, dcs, dc_leaf));
+                                 ctx.dcQn, dcs, dc_leaf, ctx.xev[3], kDcGateFrom(n)));
         }
         else
             CUEV_TRY(dc_solve<T>(*ctx.dc, s, n, ctx.h_d, ctx.h_e, evec, n, eval, A, ws.M, ctx.dcQn, gs,

@@ -87,7 +87,7 @@ template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k,
         ws.bt_blocks = bt_off[ws.bt_groups];
     }
     const size_t oBtT = off;
-    off += al((size_t)ws.bt_blocks * bt_pack_values(b) * S);
+    off += al((size_t)(ws.bt_blocks + 1) * bt_pack_values(b) * S); // + an all-zero block
     const size_t oBtO = off;
     off += al((bt_off.size() + 1) * sizeof(i64));
     ws.pool_bytes = off;
