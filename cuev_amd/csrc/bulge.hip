@@ -101,6 +101,22 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         if (c1 < c0) return;
         const int cnt = (c1 - c0 + 1) * ldb;
         T *dst = Bp + (i64)c0 * ldb;
+        if (cnt <= NEW_PER_THREAD * kBCThreads) {
+            // up to b columns (every hop): all LDS reads before the stores
+            T v[NEW_PER_THREAD];
+#pragma unroll
+            for (int u = 0; u < NEW_PER_THREAD; ++u) {
+                const int idx = tid + u * kBCThreads;
+                const int jj = idx / ldb, d = idx - jj * ldb;
+                v[u] = win[((c0 + jj) & (NSLOT - 1)) * WLD + d];
+            }
+#pragma unroll
+            for (int u = 0; u < NEW_PER_THREAD; ++u) {
+                const int idx = tid + u * kBCThreads;
+                if (idx < cnt) dst[idx] = v[u];
+            }
+            return;
+        }
         for (int idx = tid; idx < cnt; idx += kBCThreads) {
             const int jj = idx / ldb, d = idx - jj * ldb;
             dst[idx] = win[((c0 + jj) & (NSLOT - 1)) * WLD + d];
@@ -205,15 +221,29 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 const int SL = (L + NSEG - 1) / NSEG;
                 const int kb = seg * SL, ke = min(L, kb + SL), len = ke - kb;
                 T acc = T(0);
+                constexpr int FL = B / NSEG; // segment length of a full hop
                 if (seg < NSEG && t < nrow && len > 0) {
                     const int i = cq + t;
-                    int qq = t % len;
-                    for (int j = 0; j < len; ++j) {
-                        const int q = kb + qq;
-                        const int k = r0 + q;
-                        const T a = (i >= k) ? W_(i, k) : W_(k, i);
-                        acc += a * sw[q];
-                        qq = (qq + 1 == len) ? 0 : qq + 1;
+                    if (B % NSEG == 0 && (FL & (FL - 1)) == 0 && len == FL) {
+                        // full segment: unrolled with four partial sums, so the LDS
+                        // reads of consecutive steps overlap (same rotation as below)
+                        T a4[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 8
+                        for (int j = 0; j < FL; ++j) {
+                            const int q = kb + ((t + j) & (FL - 1));
+                            const int k = r0 + q;
+                            a4[j & 3] += ((i >= k) ? W_(i, k) : W_(k, i)) * sw[q];
+                        }
+                        acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+                    } else {
+                        int qq = t % len;
+                        for (int j = 0; j < len; ++j) {
+                            const int q = kb + qq;
+                            const int k = r0 + q;
+                            const T a = (i >= k) ? W_(i, k) : W_(k, i);
+                            acc += a * sw[q];
+                            qq = (qq + 1 == len) ? 0 : qq + 1;
+                        }
                     }
                 }
                 if (seg < NSEG && t < 3 * B) spart[seg * (3 * B) + t] = acc;
@@ -253,20 +283,46 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 const T wa = (a < L) ? sw[a] : T(0);
                 const T ya = (a < L) ? sp[o + a] - kap * wa : T(0);
                 const T y3 = (a < m3) ? sp[o + L + a] : T(0);
-                if (a < L) {
-                    for (int cc = cg; cc < o; cc += NG) {
-                        T &e = W_(r0 + a, cq + cc);
-                        if (cc == 0) e = (a == 0) ? (T)s_beta : T(0);
-                        else e -= wa * sp[cc];
+                constexpr bool kFull = B >= NG && B % NG == 0; // whole columns per thread group
+                if (kFull && o == B && L == B && m3 == B) {
+                    // full hop (every thread in all three regions, each element read and
+                    // written by its own thread only): every LDS read before any write,
+                    // so the round trips overlap; same arithmetic as the general path
+                    constexpr int NU = kFull ? B / NG : 1;
+                    T e1[NU], e2[NU], e3[NU], p1[NU], wc[NU], pc[NU];
+#pragma unroll
+                    for (int u = 0; u < NU; ++u) {
+                        const int cc = cg + NG * u;
+                        e1[u] = W_(r0 + a, cq + cc);
+                        p1[u] = sp[cc];
+                        e2[u] = W_(r0 + a, r0 + min(cc, a));
+                        wc[u] = sw[cc];
+                        pc[u] = sp[o + cc];
+                        e3[u] = W_(r1 + 1 + a, r0 + cc);
                     }
-                    for (int cc = cg; cc <= a; cc += NG) {
-                        const T wc = sw[cc];
-                        const T yc = sp[o + cc] - kap * wc;
-                        W_(r0 + a, r0 + cc) -= wa * yc + ya * wc;
+#pragma unroll
+                    for (int u = 0; u < NU; ++u) {
+                        const int cc = cg + NG * u;
+                        W_(r0 + a, cq + cc) = (cc == 0) ? ((a == 0) ? (T)s_beta : T(0)) : e1[u] - wa * p1[u];
+                        if (cc <= a) W_(r0 + a, r0 + cc) = e2[u] - (wa * (pc[u] - kap * wc[u]) + ya * wc[u]);
+                        W_(r1 + 1 + a, r0 + cc) = e3[u] - y3 * wc[u];
                     }
-                }
-                if (a < m3) {
-                    for (int cc = cg; cc < L; cc += NG) W_(r1 + 1 + a, r0 + cc) -= y3 * sw[cc];
+                } else {
+                    if (a < L) {
+                        for (int cc = cg; cc < o; cc += NG) {
+                            T &e = W_(r0 + a, cq + cc);
+                            if (cc == 0) e = (a == 0) ? (T)s_beta : T(0);
+                            else e -= wa * sp[cc];
+                        }
+                        for (int cc = cg; cc <= a; cc += NG) {
+                            const T wc = sw[cc];
+                            const T yc = sp[o + cc] - kap * wc;
+                            W_(r0 + a, r0 + cc) -= wa * yc + ya * wc;
+                        }
+                    }
+                    if (a < m3) {
+                        for (int cc = cg; cc < L; cc += NG) W_(r1 + 1 + a, r0 + cc) -= y3 * sw[cc];
+                    }
                 }
             }
             __syncthreads();
