@@ -123,8 +123,27 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         }
     };
 
+    // Householder vector of x = A[r0_ : r0_ + L_ - 1, c_] (column c_ of the window),
+    // by wave 0 alone: w into sw, beta into s_beta
+    auto householder = [&](int r0_, int c_, int L_) {
+        const double x = (lane < L_) ? (double)W_(r0_ + lane, c_) : 0.0;
+        const double alpha = __shfl(x, 0, 64);
+        const double tail = wave_sum((lane >= 1) ? x * x : 0.0);
+        double wv = 0.0, beta = alpha;
+        if (tail != 0.0) {
+            const double nrm = sqrt(alpha * alpha + tail);
+            beta = (alpha >= 0.0) ? -nrm : nrm;
+            const double v0 = alpha - beta;
+            const double inv = 1.0 / sqrt(v0 * v0 + tail);
+            wv = ((lane == 0) ? v0 : x) * inv;
+        }
+        if (lane < L_) sw[lane] = (T)wv;
+        if (lane == 0) s_beta = beta;
+    };
+
     for (int s = blockIdx.x; s < n - 2; s += gridDim.x) {
         int held_lo = 0, held_hi = -1; // window columns currently held in LDS
+        bool hh_ready = false;         // this hop's reflector was formed at the end of the last
         for (int h = 0;; ++h) {
             const int cq = (h == 0) ? s : s + 1 + (h - 1) * b;
             const int r0 = (h == 0) ? s + 1 : cq + b;
@@ -175,23 +194,13 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             // overlap the next sweep's start.
             if (h == 0 && !wait_and_load()) return;
 
-            // ---- Householder vector on x = A[r0:r1, cq] (wave 0)
-            if (wave == 0) {
-                const double x = (lane < L) ? (double)W_(r0 + lane, cq) : 0.0;
-                const double alpha = __shfl(x, 0, 64);
-                const double tail = wave_sum((lane >= 1) ? x * x : 0.0);
-                double wv = 0.0, beta = alpha;
-                if (tail != 0.0) {
-                    const double nrm = sqrt(alpha * alpha + tail);
-                    beta = (alpha >= 0.0) ? -nrm : nrm;
-                    const double v0 = alpha - beta;
-                    const double inv = 1.0 / sqrt(v0 * v0 + tail);
-                    wv = ((lane == 0) ? v0 : x) * inv;
-                }
-                if (lane < L) sw[lane] = (T)wv;
-                if (lane == 0) s_beta = beta;
+            // ---- Householder vector on x = A[r0:r1, cq] (wave 0), unless the previous
+            // hop formed it already (the barrier that ended that hop then covers it)
+            if (!hh_ready) {
+                if (wave == 0) householder(r0, cq, L);
+                __syncthreads();
             }
-            __syncthreads();
+            hh_ready = false;
 
             // ---- column cq is now final (rows above r0 untouched by this sweep, beta e_0
             // below): write it back and publish progress cq (release) before the rest of
@@ -216,8 +225,14 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             // (column i: stride WLD - 1 across lanes, which without the rotation puts
             // every lane on one bank)
             const int nrow = hi - cq + 1; // <= 3b
+            const int o = r0 - cq;
+            // b = 64: rows in six waves of 32 lanes, the two k segments of a row in the
+            // two halves of a wave, so p_t is completed with one lane exchange and
+            // kappa's partial sums come out of the same phase (one barrier)
+            constexpr bool kWaveSeg = 3 * B == 192 && NSEG == 2 && kBCThreads >= 384;
             {
-                const int t = tid % (3 * B), seg = tid / (3 * B);
+                const int t = kWaveSeg ? wave * 32 + (lane & 31) : tid % (3 * B);
+                const int seg = kWaveSeg ? (wave < 6 ? lane >> 5 : NSEG) : tid / (3 * B);
                 const int SL = (L + NSEG - 1) / NSEG;
                 const int kb = seg * SL, ke = min(L, kb + SL), len = ke - kb;
                 T acc = T(0);
@@ -246,12 +261,25 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                         }
                     }
                 }
-                if (seg < NSEG && t < 3 * B) spart[seg * (3 * B) + t] = acc;
+                if (kWaveSeg) {
+                    // ---- p (segment 0 + segment 1, as below), and kappa = w^T p over
+                    // the reflector's rows [o, o + L)
+                    const T other = __shfl_xor(acc, 32, 64);
+                    T kv = T(0);
+                    if (seg == 0 && t < nrow) {
+                        const T pv = T(2) * (acc + other);
+                        sp[t] = pv;
+                        if (t >= o && t < o + L) kv = pv * sw[t - o];
+                    }
+                    kv = wave_sum(kv);
+                    if (lane == 0) sred[wave] = kv;
+                } else if (seg < NSEG && t < 3 * B) {
+                    spart[seg * (3 * B) + t] = acc;
+                }
             }
             __syncthreads();
-            // ---- p, and kappa = w^T p over the reflector's rows [o, o + L)
-            const int o = r0 - cq;
-            {
+            if (!kWaveSeg) {
+                // ---- p, and kappa = w^T p over the reflector's rows [o, o + L)
                 T kv = T(0);
                 if (tid < nrow) {
                     T ps = T(0);
@@ -263,8 +291,8 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 }
                 kv = wave_sum(kv);
                 if (lane == 0) sred[wave] = kv;
+                __syncthreads();
             }
-            __syncthreads();
 
             // ---- rank-2 update A -= w y^T + y w^T with y = p - kappa w, on the three
             // regions of the window where w_i or w_j is non-zero:
@@ -333,10 +361,20 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             // releases these stores)
             const int next_cq = (h == 0) ? s + 1 : cq + b;
             const int keep_lo = min(next_cq, r1 + 1);
-            store_cols(cq + 1, keep_lo - 1);
-            held_lo = keep_lo;
             T *ucol = U + (i64)(r0 - s - 1) + (i64)s * ldu;
             for (int t = tid; t < L; t += kBCThreads) ucol[t] = sw[t];
+            // the next hop's reflector: its column next_cq is final after the rank-2
+            // update (and after sweep s-1, which waited for columns <= r1), it is not
+            // among the columns stored below, and sw / s_beta are no longer read by
+            // this hop except for U just above (wave 0's own lanes); wave 0 forms it
+            // while the other waves start the stores
+            {
+                const int nr0 = next_cq + b, nr1 = min(nr0 + b - 1, n - 1);
+                hh_ready = nr0 <= n - 1 && nr1 - nr0 + 1 >= 2;
+                if (hh_ready && wave == 0) householder(nr0, next_cq, nr1 - nr0 + 1);
+            }
+            store_cols(cq + 1, keep_lo - 1);
+            held_lo = keep_lo;
             // the next hop's loads overwrite the ring slots just stored (sweep 0 has no
             // wait phase, whose barrier would otherwise separate them)
             __syncthreads();
