@@ -26,6 +26,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include <rocblas/rocblas.h>
 
 namespace cuev {
 namespace {
@@ -361,6 +366,72 @@ size_t gemm_scratch_bytes(i64 M, i64 N, size_t elem) {
     return (size_t)3 * 512 * 128 * 128 * elem;
 }
 
+namespace {
+
+// rocBLAS handles, one per stream (a handle's workspace must not be shared by
+// work queued on two streams).
+std::mutex g_vendor_mu;
+std::vector<std::pair<hipStream_t, rocblas_handle>> g_vendor;
+
+rocblas_handle vendor_handle(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_vendor_mu);
+    for (auto &e : g_vendor)
+        if (e.first == s) return e.second;
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+    if (rocblas_set_stream(h, s) != rocblas_status_success) {
+        rocblas_destroy_handle(h);
+        return nullptr;
+    }
+    g_vendor.emplace_back(s, h);
+    return h;
+}
+
+inline rocblas_status vendor_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n,
+                                  int k, const double *alpha, const double *A, int lda, const double *B, int ldb,
+                                  const double *beta, double *C, int ldc) {
+    return rocblas_dgemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+inline rocblas_status vendor_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n,
+                                  int k, const float *alpha, const float *A, int lda, const float *B, int ldb,
+                                  const float *beta, float *C, int ldc) {
+    return rocblas_sgemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+} // namespace
+
+int gemm_vendor_attach(hipStream_t s) {
+    if (!vendor_handle(s)) CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "gemm: rocblas handle creation failed");
+    return CUEV_STATUS_SUCCESS;
+}
+
+void gemm_vendor_release(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_vendor_mu);
+    for (size_t i = 0; i < g_vendor.size(); ++i)
+        if (g_vendor[i].first == s) {
+            rocblas_destroy_handle(g_vendor[i].second);
+            g_vendor.erase(g_vendor.begin() + (long)i);
+            return;
+        }
+}
+
+namespace {
+
+template <typename T>
+int gemm_vendor(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A, const Operand<T> &B,
+                T beta, T *C, i64 ldc) {
+    rocblas_handle h = vendor_handle(s);
+    if (!h) CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "gemm: rocblas handle creation failed");
+    const rocblas_operation ta = A.trans ? rocblas_operation_transpose : rocblas_operation_none;
+    const rocblas_operation tb = B.trans ? rocblas_operation_transpose : rocblas_operation_none;
+    const rocblas_status st = vendor_gemm(h, ta, tb, (int)M, (int)N, (int)K, &alpha, A.p0, (int)A.ld0, B.p0,
+                                          (int)B.ld0, &beta, C, (int)ldc);
+    if (st != rocblas_status_success) CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "gemm: rocblas_?gemm returned status %d", (int)st);
+    return CUEV_STATUS_SUCCESS;
+}
+
+} // namespace
+
 template <typename T>
 int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
             const Operand<T> &B, T beta, T *C, i64 ldc, int lower, Scratch scratch) {
@@ -372,6 +443,10 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     if ((A.split >= 0 && A.split % BK) || (B.split >= 0 && B.split % BK))
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: K split must be a multiple of %d", BK);
     if (ldc < M) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: ldc < M");
+    const i64 imax = 0x7fffffffLL;
+    if (scratch.vendor && !lower && A.split < 0 && B.split < 0 && !A.sym && M >= 1024 && N >= 1024 &&
+        K >= 1024 && M <= imax && N <= imax && K <= imax && A.ld0 <= imax && B.ld0 <= imax && ldc <= imax)
+        return gemm_vendor<T>(s, M, N, K, alpha, A, B, beta, C, ldc);
 
     KParams<T> p{};
     p.a0 = A.p0;

@@ -182,8 +182,11 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
             CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipEventRecord(ev[7], s));
         {
+            // a plain product with the GPU to itself: the vendor library's tiles
             prof::Scope sc(s, "bt.gemm", 2.0 * (double)n * n * n);
-            CUEV_TRY(gemm<T>(s, 1, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gs));
+            Scratch gv = gs;
+            gv.vendor = 1;
+            CUEV_TRY(gemm<T>(s, 1, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gv));
         }
         CUEV_HIP(hipEventRecord(ev[8], s));
     } else {
