@@ -8,9 +8,10 @@ all eigenvalues and eigenvectors. One step = one full cuev_dsymm_eig_solve call
 GEMM), input already resident in HBM; the copy of A that the solver consumes is
 made before each step's timer starts.
 
---only chase|copy runs one kernel of interest and nothing else (for hardware counter
-runs): one bulge chase of a random packed band of the configured n and b, or a plain
-1 GiB device copy (the calibration of the FETCH_SIZE / WRITE_SIZE counters).
+--only chase|bcback|copy runs one kernel of interest and nothing else (for hardware
+counter runs): one bulge chase of a random packed band of the configured n and b, the
+bc_back application of that chase's Q_b to the identity, or a plain 1 GiB device copy
+(the calibration of the FETCH_SIZE / WRITE_SIZE counters).
 
 Prints ONE JSON line: value = mean EVD wall seconds per solve (lower is better),
 per-stage times, correctness against the exact eigenvalues, the roofline of the
@@ -80,6 +81,26 @@ def measured_traffic(tag, n, b):
     return None
 
 
+def residual_orth(torch, A0, V, lam, cols=2048):
+    """||A V - V diag(lam)||_F / (n ||A||_F) and ||V^T V - I||_F / n, accumulated over
+    column blocks of V so that the check needs O(n * cols) extra memory (at n = 49152
+    the solver's cached workspace and three n x n matrices already fill most of HBM)."""
+    n = A0.shape[0]
+    an = torch.linalg.matrix_norm(A0).item()
+    r2 = o2 = 0.0
+    for c0 in range(0, n, cols):
+        c1 = min(n, c0 + cols)
+        Vc = V[:, c0:c1]
+        R = A0 @ Vc
+        R.sub_(Vc * lam[c0:c1])
+        r2 += torch.linalg.matrix_norm(R).item() ** 2
+        G = Vc.T @ V
+        G[:, c0:c1].diagonal().sub_(1.0)
+        o2 += torch.linalg.matrix_norm(G).item() ** 2
+        del R, G
+    return r2 ** 0.5 / (n * an), o2 ** 0.5 / n
+
+
 def run_only(which, n, b, k):
     """One dispatch of interest (after one untimed repeat), for counter collection."""
     import torch
@@ -109,6 +130,19 @@ def run_only(which, n, b, k):
             torch.cuda.synchronize()
             dt = time.perf_counter() - t
             assert st == 0, cv.lib().cuev_last_error_message()
+        if which == "bcback":
+            # the chase has left its reflectors in the handle's U: apply Q_b to I
+            del band, B
+            M = torch.eye(n, dtype=torch.float64, device="cuda")
+            for _ in range(2):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                h.bc_back(M, n)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t
+            print(json.dumps({"only": "bcback", "n": n, "b": b, "seconds": dt,
+                              "tflops": 2.0 * n ** 3 / dt / 1e12}))
+            return
     hops = n * n / (2.0 * b)
     print(json.dumps({"only": "chase", "n": n, "b": b, "seconds": dt,
                       "algorithmic_bytes": hops * 4 * b * b * 8}))
@@ -129,8 +163,9 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="also time the mixed FP32+refinement form")
     ap.add_argument("--cpu-n", type=int, default=6144, help="n of the dsyevd CPU sample (0: skip)")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--only", choices=["chase", "copy"], default=None,
-                    help="run one kernel of interest and exit (hardware counter runs)")
+    ap.add_argument("--only", choices=["chase", "bcback", "copy"], default=None,
+                    help="run one kernel of interest and exit (hardware counter runs); bcback "
+                         "applies the Q_b of one chase to the identity, alone on the GPU")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -192,14 +227,7 @@ def main():
     if not a.no_check:
         lam64 = lam.double()
         Vd = V.double()
-        An = torch.linalg.matrix_norm(A0).item()
-        R = A0 @ Vd - Vd * lam64
-        res = torch.linalg.matrix_norm(R).item() / (n * An)
-        del R
-        G = Vd.T @ Vd
-        G.diagonal().sub_(1.0)
-        orth = torch.linalg.matrix_norm(G).item() / n
-        del G
+        res, orth = residual_orth(torch, A0, Vd, lam64)
         a2 = lam64.abs().max().item()
         check = {"residual": res, "orthogonality": orth,
                  "residual_bench_solver_form": res * n}
@@ -258,11 +286,16 @@ def main():
             for i in range(a.steps):
                 dt, lam_m, V_m, tm_m = one("mixed")
                 mt.append(dt)
-            Vd = V_m.double()
-            R = A0 @ Vd - Vd * lam_m.double()
-            res_m = torch.linalg.matrix_norm(R).item() / (n * torch.linalg.matrix_norm(A0).item())
+            res_m, orth_m = residual_orth(torch, A0, V_m.double(), lam_m.double())
             mixed = {"seconds": sum(mt) / len(mt), "speedup_vs_fp64": mean_s / (sum(mt) / len(mt)),
-                     "residual": res_m, "refine_sweeps": tm_m["refine_sweeps"]}
+                     "residual": res_m, "orthogonality": orth_m,
+                     "refine_sweeps": tm_m["refine_sweeps"],
+                     "stages_ms": {kk: round(float(v), 2) for kk, v in tm_m.items()
+                                   if kk.endswith("_ms")}}
+            if lam_exact is not None:
+                mixed["eig_err_vs_exact_over_norm2"] = \
+                    (lam_m.double() - lam_exact).abs().max().item() / lam_exact.abs().max().item()
+            log(f"[bench] mixed {mixed}")
         except cv.CuevError as e:
             mixed = {"error": str(e)}
 
@@ -274,7 +307,7 @@ def main():
         log(f"[bench] CPU baseline {cpu}")
 
     out = {
-        "metric": "EVD wall s at n=16k fp64 (two-stage, all eigenpairs)",
+        "metric": f"EVD wall s at n={n} fp64 (two-stage, all eigenpairs)",
         "value": round(mean_s, 4),
         "unit": "s",
         "n_gpus": 1,
