@@ -29,11 +29,14 @@ def test_mixed_random_symmetric(torch_dev, n):
     assert ev < 1e-12 and res < 1e-13 and orth < 1e-13, (ev, res, orth, t)
 
 
-def test_mixed_geometric_with_cluster(torch_dev):
-    """Scaled-down mixed16k: geometric spectrum 1..1e-6, random signs, 10% in a 1e-9 cluster."""
+@pytest.mark.parametrize("n", [500, 2048])
+def test_mixed_geometric_with_cluster(torch_dev, n):
+    """Scaled-down mixed16k: geometric spectrum 1..1e-6, random signs, 10% in a 1e-9 cluster.
+    At n=2048 more than 512 columns are clustered, so the Rayleigh-Ritz step runs on the
+    GPU (one FP64 solve of the projected union); at n=500 (fewer than 512 columns in all)
+    it runs per cluster on the host."""
     torch = torch_dev
     from cuev_amd import workloads
-    n = 2048
     Q = workloads.orthogonal_q(torch, n, 1, "cuda", torch.float64)
     lam_x = workloads.lam_geometric_cluster(torch, n, 103, "cuda")
     A = workloads.sandwich(torch, Q, lam_x)
@@ -43,6 +46,7 @@ def test_mixed_geometric_with_cluster(torch_dev):
     lam_n, Vn = lam.cpu().numpy(), V.cpu().numpy()
     ev, res, orth = eig_metrics(An, lam_n, Vn, lam_x.cpu().numpy())
     assert ev < 1e-12 and res < 1e-13 and orth < 1e-13, (ev, res, orth, t)
+    assert (t["n_cluster_cols"] >= 512) == (n == 2048), t
 
 
 def test_mixed_rejected_for_float(torch_dev):
