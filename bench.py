@@ -134,14 +134,16 @@ def run_only(which, n, b, k):
             # the chase has left its reflectors in the handle's U: apply Q_b to I
             del band, B
             M = torch.eye(n, dtype=torch.float64, device="cuda")
-            for _ in range(2):
-                torch.cuda.synchronize()
-                t = time.perf_counter()
-                h.bc_back(M, n)
-                torch.cuda.synchronize()
-                dt = time.perf_counter() - t
-            print(json.dumps({"only": "bcback", "n": n, "b": b, "seconds": dt,
-                              "tflops": 2.0 * n ** 3 / dt / 1e12}))
+            res = {"only": "bcback", "n": n, "b": b}
+            for name, f in (("forward", h.bc_back), ("transposed", h.bc_back_t)):
+                for _ in range(2):
+                    torch.cuda.synchronize()
+                    t = time.perf_counter()
+                    f(M, n)
+                    torch.cuda.synchronize()
+                    dt = time.perf_counter() - t
+                res[name] = {"seconds": dt, "tflops": 2.0 * n ** 3 / dt / 1e12}
+            print(json.dumps(res))
             return
     hops = n * n / (2.0 * b)
     print(json.dumps({"only": "chase", "n": n, "b": b, "seconds": dt,
@@ -165,7 +167,7 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["chase", "bcback", "copy"], default=None,
                     help="run one kernel of interest and exit (hardware counter runs); bcback "
-                         "applies the Q_b of one chase to the identity, alone on the GPU")
+                         "applies the Q_b of one chase (and its transpose) to the identity, alone on the GPU")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -283,15 +285,22 @@ def main():
             for i in range(a.warmup):
                 one("mixed")
             mt = []
+            cv.profile_enable(True)
+            cv.profile_read(reset=True)
             for i in range(a.steps):
                 dt, lam_m, V_m, tm_m = one("mixed")
                 mt.append(dt)
+            mprof = cv.profile_read(reset=True)
+            cv.profile_enable(False)
             res_m, orth_m = residual_orth(torch, A0, V_m.double(), lam_m.double())
             mixed = {"seconds": sum(mt) / len(mt), "speedup_vs_fp64": mean_s / (sum(mt) / len(mt)),
                      "residual": res_m, "orthogonality": orth_m,
                      "refine_sweeps": tm_m["refine_sweeps"],
                      "stages_ms": {kk: round(float(v), 2) for kk, v in tm_m.items()
-                                   if kk.endswith("_ms")}}
+                                   if kk.endswith("_ms")},
+                     # library events per mixed solve (the Rayleigh-Ritz step's inner FP64
+                     # solve shows under the FP64 stage names)
+                     "kernels_ms": {t: round(r["ms"] / len(mt), 2) for t, r in mprof.items()}}
             if lam_exact is not None:
                 mixed["eig_err_vs_exact_over_norm2"] = \
                     (lam_m.double() - lam_exact).abs().max().item() / lam_exact.abs().max().item()

@@ -95,6 +95,7 @@ def _declare(L):
         getattr(L, f"cuev_{pre}_tridi_dc").argtypes = [P, P, P, P, P]
         getattr(L, f"cuev_{pre}_back_transform").argtypes = [P, P, P, P, P, P]
         getattr(L, f"cuev_{pre}_bc_back").argtypes = [P, P, P, I64]
+        getattr(L, f"cuev_{pre}_bc_back_t").argtypes = [P, P, P, I64]
         getattr(L, f"cuev_{pre}_sbr_back").argtypes = [P, P, P, P, I64]
         sc = D if pre == "d" else F
         getattr(L, f"cuev_{pre}gemm").argtypes = [P, I, I, I64, I64, I64, sc, P, I64, P, I64, sc, P, I64]
@@ -278,6 +279,11 @@ class Handle:
     def bc_back(self, M, ldm):
         U, _ = self.buffer("U")
         _check(self._f("bc_back")(self.h, ctypes.c_void_p(U), _ptr(M), ldm))
+
+    def bc_back_t(self, M, ldm):
+        """M <- Q_b^T M (the reordered back-transform's application)."""
+        U, _ = self.buffer("U")
+        _check(self._f("bc_back_t")(self.h, ctypes.c_void_p(U), _ptr(M), ldm))
 
     def sbr_back(self, M, ldm):
         Y, _ = self.buffer("Y")

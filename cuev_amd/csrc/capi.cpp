@@ -174,6 +174,14 @@ template <typename T> int stage_bc_back(cuev_handle_t h, const T *U, T *M, i64 l
     if (!U || !M) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_back: null pointer");
     return bc_back<T>(ctx->ws, U, M, ldm);
 }
+template <typename T> int stage_bc_back_t(cuev_handle_t h, const T *U, T *M, i64 ldm) {
+    detail::clear_error();
+    CUEV_CTX(T, h);
+    if (!U || !M) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "bc_back_t: null pointer");
+    CUEV_TRY(bc_back_t<T>(ctx->ws, ctx->ws.stream, U, M, ldm));
+    CUEV_HIP(hipStreamSynchronize(ctx->ws.stream));
+    return CUEV_STATUS_SUCCESS;
+}
 template <typename T> int stage_sbr_back(cuev_handle_t h, const T *Y, const T *W, T *M, i64 ldm) {
     detail::clear_error();
     CUEV_CTX(T, h);
@@ -328,6 +336,8 @@ int cuev_s_back_transform(cuev_handle_t h, const float *Y, const float *W, const
 int cuev_d_back_transform(cuev_handle_t h, const double *Y, const double *W, const double *U, double *evec, cuev_solve_timer *t) { return stage_bt<double>(h, Y, W, U, evec, t); }
 int cuev_s_bc_back(cuev_handle_t h, const float *U, float *M, int64_t ldm) { return stage_bc_back<float>(h, U, M, ldm); }
 int cuev_d_bc_back(cuev_handle_t h, const double *U, double *M, int64_t ldm) { return stage_bc_back<double>(h, U, M, ldm); }
+int cuev_s_bc_back_t(cuev_handle_t h, const float *U, float *M, int64_t ldm) { return stage_bc_back_t<float>(h, U, M, ldm); }
+int cuev_d_bc_back_t(cuev_handle_t h, const double *U, double *M, int64_t ldm) { return stage_bc_back_t<double>(h, U, M, ldm); }
 int cuev_s_sbr_back(cuev_handle_t h, const float *Y, const float *W, float *M, int64_t ldm) { return stage_sbr_back<float>(h, Y, W, M, ldm); }
 int cuev_d_sbr_back(cuev_handle_t h, const double *Y, const double *W, double *M, int64_t ldm) { return stage_sbr_back<double>(h, Y, W, M, ldm); }
 

@@ -12,6 +12,7 @@
 //    to the identity); after the D&C only one GEMM evec = X Q_d remains.
 #include "solver.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -49,9 +50,13 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 // Smallest merge of the device D&C that waits for the back-transform running beside
-// it: the three top tree levels, which hold nearly all of the D&C's GEMM work
-// (measured at n = 16384 against other levels, no gate, and no overlap at all).
-int kDcGateFrom(int n) { return n / 4; }
+// it: the three top tree levels at n = 16384, which hold nearly all of the D&C's GEMM
+// work (measured there against other levels, no gate, and no overlap at all), and
+// every merge of more than 4096 rows at larger n, so the GEMM-heavy levels never share
+// the GPU with bc_back (merges of up to 4096 rows leave it at its own speed at
+// n = 16384). At n = 49152 one solve took 25.9 s with this gate and 27.8 s with the
+// top three levels only, but bc_back alone varies there from 7.6 to 12 s between runs.
+int kDcGateFrom(int n) { return std::min(n / 4, 4096); }
 } // namespace
 
 template <typename T>

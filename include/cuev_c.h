@@ -23,6 +23,8 @@
  *   cuev_{s,d}_tridi_dc           include/cuda/kernels.cuh:114, src/cuda/tridi.cu:425-452
  *   cuev_{s,d}_back_transform     include/cuda/kernels.cuh:135-137, src/cuda/backtransform.cu:232-275
  *   cuev_{s,d}_bc_back            include/cuda/kernels.cuh:144, src/cuda/backtransform.cu:183-207
+ *   cuev_{s,d}_bc_back_t          no counterpart: the transposed application of the reordered
+ *                                 back-transform (listed as future work, README.md:35)
  *   cuev_{s,d}_sbr_back           include/cuda/kernels.cuh:152, src/cuda/backtransform.cu:210-230
  *   cuev_{s,d}gemm / symm / syr2k include/cuda/kernels.cuh:162-283, src/cuda/cublas.cu:15-56
  *                                 (vendor BLAS call sites, here served by our own MFMA kernels)
@@ -209,6 +211,10 @@ int cuev_d_back_transform(cuev_handle_t h, const double *Y, const double *W, con
 /* M <- Q_b M, M is n x n with leading dimension ldm >= n. */
 int cuev_s_bc_back(cuev_handle_t h, const float *U, float *M, int64_t ldm);
 int cuev_d_bc_back(cuev_handle_t h, const double *U, double *M, int64_t ldm);
+/* M <- Q_b^T M (no reference counterpart: the reordered back-transform builds
+ * Q_b^T Q_s^T with it); M is n x n with leading dimension ldm >= n. */
+int cuev_s_bc_back_t(cuev_handle_t h, const float *U, float *M, int64_t ldm);
+int cuev_d_bc_back_t(cuev_handle_t h, const double *U, double *M, int64_t ldm);
 /* M <- Q_s M, M is n x n with leading dimension ldm >= n. */
 int cuev_s_sbr_back(cuev_handle_t h, const float *Y, const float *W, float *M, int64_t ldm);
 int cuev_d_sbr_back(cuev_handle_t h, const double *Y, const double *W, double *M, int64_t ldm);

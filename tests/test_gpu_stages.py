@@ -224,11 +224,16 @@ def test_back_transform_factors(torch_dev, n, b, k):
     h.bc_back(M, n)
     torch.cuda.synchronize()
     Qb = from_dev(M)
+    M = to_dev(np.eye(n))
+    h.bc_back_t(M, n)  # the reordered order's transposed application
+    torch.cuda.synchronize()
+    QbT = from_dev(M)
     h.close()
     Qs_o = O.sbr_back(np.asfortranarray(Y), np.asfortranarray(W), np.eye(n, order="F"), b)
     Qb_o = O.bc_back(np.asfortranarray(U), np.eye(n, order="F"), b)
     assert np.abs(Qs - Qs_o).max() < 1e-12
     assert np.abs(Qb - Qb_o).max() < 1e-12
+    assert np.abs(QbT - Qb_o.T).max() < 1e-12
     # the reference's invariants
     assert np.linalg.norm(Qs.T @ As @ Qs - Band) / np.linalg.norm(Band) < 1e-9
     assert np.linalg.norm(Qb.T @ Band @ Qb - Tri) / np.linalg.norm(Tri) < 1e-9
