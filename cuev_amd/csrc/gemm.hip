@@ -444,8 +444,8 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: K split must be a multiple of %d", BK);
     if (ldc < M) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: ldc < M");
     const i64 imax = 0x7fffffffLL;
-    if (scratch.vendor && !lower && A.split < 0 && B.split < 0 && !A.sym && M >= 1024 && N >= 1024 &&
-        K >= 1024 && M <= imax && N <= imax && K <= imax && A.ld0 <= imax && B.ld0 <= imax && ldc <= imax)
+    if (scratch.vendor && !lower && A.split < 0 && B.split < 0 && !A.sym && std::min(std::min(M, N), K) >= 512 &&
+        (double)M * N * K >= 8.6e9 && M <= imax && N <= imax && K <= imax && A.ld0 <= imax && B.ld0 <= imax && ldc <= imax)
         return gemm_vendor<T>(s, M, N, K, alpha, A, B, beta, C, ldc);
 
     KParams<T> p{};

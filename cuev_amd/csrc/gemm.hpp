@@ -50,9 +50,9 @@ inline Operand<T> opnd_cat(const T *p0, i64 ld0, const T *p1, i64 ld1, i64 split
 // only 64 x 64 block tiles (about 100 VGPRs instead of about 220), for products
 // meant to run beside a kernel that holds most of each SIMD's registers. vendor:
 // a large plain product (no K-concatenation, no symmetric operand, all of C, every
-// dimension >= 1024) goes to rocBLAS, which runs such shapes at 73-76 TF/s FP64
-// on gfx950 against about 62 for this engine; for products that have the GPU to
-// themselves.
+// dimension >= 512, M N K >= 8.6e9) goes to rocBLAS, which runs such shapes at
+// 67-76 TF/s FP64 on gfx950 against 50-62 for this engine (n^3 products: 75 vs 62;
+// rank-512 updates of an n x n matrix: 67-69 vs 50-53).
 struct Scratch {
     void *ptr = nullptr;
     size_t bytes = 0;
