@@ -150,12 +150,59 @@ def run_only(which, n, b, k):
                       "algorithmic_bytes": hops * 4 * b * b * 8}))
 
 
+def run_diff(n, steps, warmup):
+    """Forward and backward eigh on the differentiation workload (A = Q diag(lam) Q^T, lam
+    equispaced in [-1, 1]) in fp64 and fp32. Cotangents of L = sum_i w_i lam_i +
+    ||V_k^T B||_F^2 (k = 128, B n x 64): lambar = w, Vbar[:, :k] = 2 B B^T V_k. Backward
+    work = three n^3 products (V^T Vbar, V K, (V K) V^T) = 6 n^3 flops."""
+    import torch
+    import cuev_amd as cv
+    from cuev_amd import workloads
+    A0, lam_exact = workloads.make("diff4k", n=n, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    w = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+    B = torch.randn((n, 64), generator=g, dtype=torch.float64, device="cuda")
+    k = 128
+    out = {"metric": f"eigh forward + backward wall ms at n={n}", "n": n, "k": k}
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        A = A0.to(dt)
+        fw, bw = [], []
+        for i in range(warmup + steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lam, V = cv.symm_eig_solve(A)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            Vk = V[:, :k]
+            Vbar = torch.zeros_like(V)
+            Vbar[:, :k] = 2.0 * (B.to(dt) @ (B.to(dt).T @ Vk))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            Abar, masked = cv.eig_backward(V, lam, Vbar, w.to(dt), gap_tol=1e-6 if dt == torch.float64 else 1e-5)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            if i >= warmup:
+                fw.append(t1 - t0)
+                bw.append(t3 - t2)
+        fwd = sum(fw) / len(fw)
+        bwd = sum(bw) / len(bw)
+        asym = (Abar - Abar.T).abs().max().item() / Abar.abs().max().item()
+        err = (lam.double() - lam_exact).abs().max().item()
+        out[name] = {"forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
+                     "backward_tflops": round(6.0 * n ** 3 / bwd / 1e12, 2), "masked_pairs": masked,
+                     "abar_asymmetry": asym, "eig_err_vs_exact": err}
+        log(f"[bench] diff {name}: {out[name]}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="q16k", choices=["q16k", "n1024", "g49k", "mixed16k"])
+    ap.add_argument("--workload", default="q16k", choices=["q16k", "n1024", "g49k", "mixed16k", "diff4k"],
+                    help="diff4k: forward + backward eigh at n=4096 in fp64 and fp32 (own JSON line)")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--b", type=int, default=64)
     ap.add_argument("--k", type=int, default=512)
@@ -181,11 +228,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    n_default = {"q16k": 16384, "n1024": 1024, "g49k": 49152, "mixed16k": 16384}[a.workload]
+    n_default = {"q16k": 16384, "n1024": 1024, "g49k": 49152, "mixed16k": 16384, "diff4k": 4096}[a.workload]
     n = a.n or n_default
     b, k = a.b, a.k
     if a.only:
         run_only(a.only, n, b, k)
+        return
+    if a.workload == "diff4k":
+        run_diff(a.n or 4096, a.steps, a.warmup)
         return
     if a.workload == "n1024" and not a.n:
         b, k = 32, 256

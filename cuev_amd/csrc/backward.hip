@@ -11,8 +11,8 @@
 //
 // Products: C = V^T Vbar, then K = diag(lambar) + F o skew(C) in one fused
 // element-wise pass over pairs (i < j) that writes K[i,j] and K[j,i] in place,
-// then T = V K and Abar = T V^T — all on the MFMA engine. Only V and lambda
-// are needed from the forward pass.
+// then T = V K and Abar = T V^T (rocBLAS for large n, the MFMA tile engine
+// below that). Only V and lambda are needed from the forward pass.
 #include "backward.hpp"
 
 #include "gemm.hpp"
@@ -70,6 +70,7 @@ int eig_backward(const T *V, const T *lam, const T *Vbar, const T *lambar, int n
         CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "eig_backward: device allocation failed (n=%d)", n);
     }
     Scratch gs{scr, scr_bytes};
+    gs.vendor = 1; // three plain n^3 products with the GPU to themselves: rocBLAS when large
     int st = CUEV_STATUS_SUCCESS;
     do {
         if ((st = fill_int(s, dmask, 1, 0))) break;
