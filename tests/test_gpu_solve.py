@@ -172,3 +172,26 @@ def test_reordered_repeatable_n16384(torch_dev):
             first = lam.clone()
         else:
             assert torch.equal(lam, first)
+
+
+def test_reordered_above_one_round_n16500(torch_dev):
+    """n=16500 is more columns than one resident round of the bc_back kernel covers
+    (16384 at one wave per SIMD), so Q_b^T is applied in two column chunks. Known
+    spectrum; residual and orthogonality at the north-star tolerances, computed on the
+    GPU in float64."""
+    torch = torch_dev
+    from cuev_amd import workloads
+    n = 16500
+    A0, lam_exact = workloads.make("q16k", n=n, device="cuda")
+    lam, V = cv.symm_eig_solve(A0, bandwidth=64, block=512, bt_mode="reordered", dc_mode="device")
+    torch.cuda.synchronize()
+    ref = torch.sort(lam_exact.double()).values
+    assert (torch.sort(lam.double()).values - ref).abs().max().item() <= 1e-12 * ref.abs().max().item()
+    V = V.double()
+    R = A0 @ V - V * lam.double()
+    res = torch.linalg.matrix_norm(R).item() / (n * torch.linalg.matrix_norm(A0).item())
+    del R
+    G = V.T @ V
+    G.diagonal().sub_(1.0)
+    orth = torch.linalg.matrix_norm(G).item() / n
+    assert res <= 1e-13 and orth <= 1e-13, (res, orth)
