@@ -16,8 +16,15 @@ bc_back application of that chase's Q_b to the identity, or a plain 1 GiB device
 Prints ONE JSON line: value = mean EVD wall seconds per solve (lower is better),
 per-stage times, correctness against the exact eigenvalues, the roofline of the
 dominant GPU kernel measured with HIP events inside the library, and host LAPACK
-dsyevd on a bounded sample as the CPU baseline. With --mixed the FP32 + FP64
-refinement form is timed on the same matrix as well.
+dsyevd as the CPU baseline (default: on the very n = 16384 matrix the GPU solved,
+copied to the host). With --mixed the FP32 + FP64 refinement form is timed on the
+same matrix as well.
+
+The default q16k run adds the metric's other two numbers as extra keys (skip them
+with --no-extras): "g49k", one timed n = 49152 solve after one untimed one, with its
+accuracy checked over column blocks of V; and "mixed", the mixed16k matrix (n = 16384,
+geometric clustered spectrum) solved by the FP64 path and by the mixed FP32 +
+refinement path, with both times, the speedup and the accuracy of each.
 """
 import argparse
 import json
@@ -37,16 +44,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_sample, n_full, seed=1):
-    """Host LAPACK dsyevd (SciPy's OpenBLAS) on an n_sample matrix of the same construction."""
-    import numpy as np
-    import scipy.linalg as sl
+def _blas_threads():
     try:
         from threadpoolctl import threadpool_info
-        threads = max([d.get("num_threads", 0) for d in threadpool_info()
-                       if d.get("internal_api") == "openblas"] or [os.cpu_count()])
+        return max([d.get("num_threads", 0) for d in threadpool_info()
+                    if d.get("internal_api") == "openblas"] or [os.cpu_count()])
     except Exception:
-        threads = os.cpu_count()
+        return os.cpu_count()
+
+
+def cpu_baseline_same(A_host, lam_exact=None):
+    """Host LAPACK dsyevd (jobz=V, uplo=L; SciPy's OpenBLAS, all its threads) on the very
+    matrix the GPU solved, copied to the host; generation and the copy are excluded."""
+    import numpy as np
+    import scipy.linalg as sl
+    n = A_host.shape[0]
+    A = np.asfortranarray(A_host)
+    t0 = time.perf_counter()
+    w, V = sl.eigh(A, lower=True, driver="evd", overwrite_a=True, check_finite=False)
+    t = time.perf_counter() - t0
+    out = {"value": round(t, 3), "unit": "s", "cores": int(_blas_threads()), "kind": "port",
+           "sample": f"LAPACK dsyevd (jobz=V, uplo=L, SciPy OpenBLAS) on the same n={n} fp64 "
+                     f"matrix the GPU solved (copied to the host), measured once, not scaled"}
+    if lam_exact is not None:
+        le = np.sort(np.asarray(lam_exact))
+        out["eig_err_vs_exact_over_norm2"] = float(np.abs(w - le).max() / np.abs(le).max())
+    return out
+
+
+def cpu_baseline(n_sample, n_full, seed=1):
+    """Host LAPACK dsyevd (SciPy's OpenBLAS) on an n_sample matrix of the same construction,
+    scaled by (n_full / n_sample)^3 (used where the full size would take too long)."""
+    import numpy as np
+    import scipy.linalg as sl
+    threads = _blas_threads()
     rng = np.random.default_rng(seed)
     Q, _ = np.linalg.qr(rng.standard_normal((n_sample, n_sample)))
     lam = np.sort(rng.uniform(-1.0, 1.0, n_sample))
@@ -82,23 +113,8 @@ def measured_traffic(tag, n, b):
 
 
 def residual_orth(torch, A0, V, lam, cols=2048):
-    """||A V - V diag(lam)||_F / (n ||A||_F) and ||V^T V - I||_F / n, accumulated over
-    column blocks of V so that the check needs O(n * cols) extra memory (at n = 49152
-    the solver's cached workspace and three n x n matrices already fill most of HBM)."""
-    n = A0.shape[0]
-    an = torch.linalg.matrix_norm(A0).item()
-    r2 = o2 = 0.0
-    for c0 in range(0, n, cols):
-        c1 = min(n, c0 + cols)
-        Vc = V[:, c0:c1]
-        R = A0 @ Vc
-        R.sub_(Vc * lam[c0:c1])
-        r2 += torch.linalg.matrix_norm(R).item() ** 2
-        G = Vc.T @ V
-        G[:, c0:c1].diagonal().sub_(1.0)
-        o2 += torch.linalg.matrix_norm(G).item() ** 2
-        del R, G
-    return r2 ** 0.5 / (n * an), o2 ** 0.5 / n
+    from cuev_amd import workloads
+    return workloads.residual_orth(A0, V, lam, cols)
 
 
 def run_only(which, n, b, k):
@@ -210,7 +226,12 @@ def main():
     ap.add_argument("--dc", default="auto", choices=["auto", "host", "device"],
                     help="tridiagonal D&C: host LAPACK dstedc or the device-resident form")
     ap.add_argument("--mixed", action="store_true", help="also time the mixed FP32+refinement form")
-    ap.add_argument("--cpu-n", type=int, default=6144, help="n of the dsyevd CPU sample (0: skip)")
+    ap.add_argument("--cpu", default="same", choices=["same", "sample", "none"],
+                    help="CPU baseline: dsyevd on the same matrix (default), on an n=--cpu-n sample "
+                         "scaled by n^3, or none")
+    ap.add_argument("--cpu-n", type=int, default=6144, help="n of the dsyevd CPU sample for --cpu sample")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="q16k: skip the n=49152 solve and the mixed16k FP64/mixed pair")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["chase", "bcback", "copy"], default=None,
                     help="run one kernel of interest and exit (hardware counter runs); bcback "
@@ -245,14 +266,16 @@ def main():
     torch.cuda.synchronize()
     log(f"[bench] generated in {time.perf_counter() - t0:.1f} s; lapack={cv.lapack_name()}")
 
+    # column-major buffer (strides (1, n)) consumed in place by the solver; A0 is exactly
+    # symmetric, so filling it through its transposed view copies A0 itself
     work = torch.empty_like(A0)
 
     def one(precision="native", timer=True):
-        work.copy_(A0)
+        work.mT.copy_(A0)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        lam, V, tm = cv.symm_eig_solve(work, overwrite=True, bandwidth=b, block=k, bt_mode=a.bt, dc_mode=a.dc,
-                                       precision=precision, timer=True)
+        lam, V, tm = cv.symm_eig_solve(work.mT, overwrite=True, bandwidth=b, block=k, bt_mode=a.bt,
+                                       dc_mode=a.dc, precision=precision, timer=True)
         torch.cuda.synchronize()
         return time.perf_counter() - t, lam, V, tm
 
@@ -278,14 +301,13 @@ def main():
     check = {}
     if not a.no_check:
         lam64 = lam.double()
-        Vd = V.double()
-        res, orth = residual_orth(torch, A0, Vd, lam64)
+        res, orth = residual_orth(torch, A0, V, lam64)
         a2 = lam64.abs().max().item()
         check = {"residual": res, "orthogonality": orth,
                  "residual_bench_solver_form": res * n}
         if lam_exact is not None:
             check["eig_err_vs_exact_over_norm2"] = (lam64 - lam_exact).abs().max().item() / a2
-        check["pass_1e-13"] = bool(res <= 1e-13 and orth <= 1e-13)
+        check["pass_1e-13"] = bool(res <= 1e-13 and orth <= 1e-13 and res * n <= 1e-13)
         log(f"[bench] residual {res:.2e} orth {orth:.2e} {check}")
 
     # ---- per-kernel rates (library HIP events over the timed steps)
@@ -300,27 +322,7 @@ def main():
             ent["gbps"] = r["bytes"] / (r["ms"] * 1e-3) / 1e9
         kernels[tag] = ent
 
-    # dominant single-kernel launch site by GPU time per step
-    single = {"bc.chase": "hbm", "bt.bc_back": "mfma", "bt.gemm": "mfma", "dbbr.syr2k": "mfma",
-              "dbbr.symm": "mfma"}
-    dom = max((t for t in single if t in prof), key=lambda t: prof[t]["ms"], default=None)
-    roofline = None
-    if dom:
-        r = prof[dom]
-        dur = r["ms"] / r["count"] * 1e-3
-        if single[dom] == "hbm":
-            per_launch = r["bytes"] / r["count"]
-            ach = per_launch / dur / 1e9
-            roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0,
-                        "unit": "GB/s", "frac": round(ach / 8000.0, 4),
-                        "traffic": measured_traffic(dom, n, b),
-                        "avg_launch_ms": round(dur * 1e3, 3)}
-        else:
-            per_launch = r["flops"] / r["count"]
-            ach = per_launch / dur / 1e12
-            roofline = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": 78.6,
-                        "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
-                        "avg_launch_ms": round(dur * 1e3, 3)}
+    roofline = dominant_roofline(prof, n, b)
 
     # ---- stages (means over the timed steps)
     stages = {key: sum(t[key] for t in timers) / K for key in
@@ -332,36 +334,34 @@ def main():
     mixed = None
     if a.mixed:
         try:
-            for i in range(a.warmup):
-                one("mixed")
-            mt = []
-            cv.profile_enable(True)
-            cv.profile_read(reset=True)
-            for i in range(a.steps):
-                dt, lam_m, V_m, tm_m = one("mixed")
-                mt.append(dt)
-            mprof = cv.profile_read(reset=True)
-            cv.profile_enable(False)
-            res_m, orth_m = residual_orth(torch, A0, V_m.double(), lam_m.double())
-            mixed = {"seconds": sum(mt) / len(mt), "speedup_vs_fp64": mean_s / (sum(mt) / len(mt)),
-                     "residual": res_m, "orthogonality": orth_m,
-                     "refine_sweeps": tm_m["refine_sweeps"],
-                     "stages_ms": {kk: round(float(v), 2) for kk, v in tm_m.items()
-                                   if kk.endswith("_ms")},
-                     # library events per mixed solve (the Rayleigh-Ritz step's inner FP64
-                     # solve shows under the FP64 stage names)
-                     "kernels_ms": {t: round(r["ms"] / len(mt), 2) for t, r in mprof.items()}}
-            if lam_exact is not None:
-                mixed["eig_err_vs_exact_over_norm2"] = \
-                    (lam_m.double() - lam_exact).abs().max().item() / lam_exact.abs().max().item()
-            log(f"[bench] mixed {mixed}")
+            mixed = time_mixed(torch, cv, one, A0, lam_exact, a.warmup, a.steps, mean_s)
         except cv.CuevError as e:
             mixed = {"error": str(e)}
 
+    del lam, V
+    A_host = None
+    if a.cpu == "same":
+        A_host = A0.cpu().numpy()
+    del work, A0
+    cv.cache_release()
+    torch.cuda.empty_cache()
+
+    extras_on = a.workload == "q16k" and not a.no_extras and n == 16384
+    g49k = mixed16k = None
+    if extras_on:
+        g49k = run_g49k(torch, cv, workloads)
+        mixed16k = run_mixed16k(torch, cv, workloads)
+        if mixed is None:
+            mixed = mixed16k
+
     cpu = None
-    if a.cpu_n:
+    if a.cpu == "same":
+        log(f"[bench] CPU baseline: dsyevd on the same n={n} matrix")
+        cpu = cpu_baseline_same(A_host, lam_exact.cpu().numpy() if lam_exact is not None else None)
+        del A_host
+        log(f"[bench] CPU baseline {cpu}")
+    elif a.cpu == "sample" and a.cpu_n:
         log(f"[bench] CPU baseline: dsyevd n={a.cpu_n}")
-        del work
         cpu = cpu_baseline(min(a.cpu_n, n), n)
         log(f"[bench] CPU baseline {cpu}")
 
@@ -391,10 +391,145 @@ def main():
                     for t, e in kernels.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "g49k": g49k,
         "mixed": mixed,
         "lapack": cv.lapack_name(),
     }
     print(json.dumps(out), flush=True)
+
+
+def dominant_roofline(prof, n, b):
+    """Roofline of the single-launch kernel site with the most GPU time per solve.
+
+    bc_chase (HBM/L2-bound): achieved = the algorithmic bytes of the model this build's
+    library records for it (per hop b columns loaded, b stored, each b x 2b: 4 b^2 S;
+    n^2 / (2b) hops), over its HIP-event duration; "frac_model_5p5bn2S" gives the same
+    duration against the 5.5 b n^2 S L2-traffic model of a kernel that re-reads its
+    window every hop. MFMA-bound kernels: the flops the library records per launch."""
+    single = {"bc.chase": "hbm", "bt.bc_back": "mfma", "bt.gemm": "mfma", "dbbr.syr2k": "mfma",
+              "dbbr.symm": "mfma"}
+    dom = max((t for t in single if t in prof), key=lambda t: prof[t]["ms"], default=None)
+    if not dom:
+        return None
+    r = prof[dom]
+    dur = r["ms"] / r["count"] * 1e-3
+    if single[dom] == "hbm":
+        per_launch = r["bytes"] / r["count"]
+        ach = per_launch / dur / 1e9
+        alt = 5.5 * b * n * n * 8 / dur / 1e9
+        return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0,
+                "unit": "GB/s", "frac": round(ach / 8000.0, 4),
+                "traffic": measured_traffic(dom, n, b),
+                "traffic_source": "committed counter run profiles/bc_chase_q16k_pmc.txt "
+                                  "(FETCH_SIZE + WRITE_SIZE, copy-calibrated), not measured by this run",
+                "frac_model_5p5bn2S": round(alt / 8000.0, 4),
+                "avg_launch_ms": round(dur * 1e3, 3)}
+    per_launch = r["flops"] / r["count"]
+    ach = per_launch / dur / 1e12
+    return {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": 78.6,
+            "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
+            "avg_launch_ms": round(dur * 1e3, 3)}
+
+
+def time_mixed(torch, cv, one, A0, lam_exact, warmup, steps, fp64_s):
+    """The mixed FP32 + refinement form on the matrix `one` solves."""
+    for i in range(warmup):
+        one("mixed")
+    mt = []
+    cv.profile_enable(True)
+    cv.profile_read(reset=True)
+    for i in range(steps):
+        dt, lam_m, V_m, tm_m = one("mixed")
+        mt.append(dt)
+    mprof = cv.profile_read(reset=True)
+    cv.profile_enable(False)
+    res_m, orth_m = residual_orth(torch, A0, V_m, lam_m.double())
+    n = A0.shape[0]
+    ms = sum(mt) / len(mt)
+    mixed = {"seconds": round(ms, 4), "fp64_seconds": round(fp64_s, 4),
+             "speedup_vs_fp64": round(fp64_s / ms, 4),
+             "residual": res_m, "residual_bench_solver_form": res_m * n, "orthogonality": orth_m,
+             "refine_sweeps": tm_m["refine_sweeps"], "n_cluster_cols": tm_m["n_cluster_cols"],
+             "stages_ms": {kk: round(float(v), 2) for kk, v in tm_m.items() if kk.endswith("_ms")},
+             # library events per mixed solve (the Rayleigh-Ritz step's inner FP64
+             # solve shows under the FP64 stage names)
+             "kernels_ms": {t: round(r["ms"] / len(mt), 2) for t, r in mprof.items()}}
+    if lam_exact is not None:
+        mixed["eig_err_vs_exact_over_norm2"] = \
+            (lam_m.double() - lam_exact).abs().max().item() / lam_exact.abs().max().item()
+    log(f"[bench] mixed {mixed}")
+    return mixed
+
+
+def _solve_timed(torch, cv, work, A0, **kw):
+    work.mT.copy_(A0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lam, V, tm = cv.symm_eig_solve(work.mT, overwrite=True, timer=True, **kw)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t, lam, V, tm
+
+
+def run_g49k(torch, cv, workloads):
+    """One untimed and one timed fp64 solve of g49k (n = 49152, A = (G + G^T)/2, G iid
+    N(0, 1/n), seed 2, b = 64, k = 512), accuracy over column blocks of V."""
+    n = 49152
+    log(f"[bench] g49k: generating n={n}")
+    A0, _ = workloads.make("g49k", n=n, device="cuda")
+    work = torch.empty_like(A0)
+    dt0, lam, V, _ = _solve_timed(torch, cv, work, A0, bandwidth=64, block=512)
+    log(f"[bench] g49k untimed solve: {dt0:.2f} s")
+    del lam, V
+    dt, lam, V, tm = _solve_timed(torch, cv, work, A0, bandwidth=64, block=512)
+    log(f"[bench] g49k timed solve: {dt:.2f} s {tm}")
+    del work
+    res, orth = residual_orth(torch, A0, V, lam.double())
+    out = {"workload": "g49k: n=49152, b=64, k=512, A=(G+G^T)/2, G iid N(0,1/n), seed 2",
+           "seconds": round(dt, 3), "untimed_first_solve_s": round(dt0, 3),
+           "effective_tflops_16_3": round(16.0 / 3.0 * n ** 3 / dt / 1e12, 3),
+           "dbbr_frac_fp64_peak": round(4.0 / 3.0 * n ** 3 / (tm["dbbr_ms"] * 1e-3) / 78.6e12, 4),
+           "stages_ms": {kk: round(float(v), 2) for kk, v in tm.items() if kk.endswith("_ms")},
+           "residual": res, "residual_bench_solver_form": res * n, "orthogonality": orth,
+           "pass_1e-13": bool(res * n <= 1e-13 and orth <= 1e-13)}
+    log(f"[bench] g49k {out}")
+    del A0, V, lam
+    cv.cache_release()
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_mixed16k(torch, cv, workloads, steps=2):
+    """mixed16k (n = 16384, lam geometric 1..1e-6 with random signs, 10% in a 1e-9
+    cluster): the FP64 path and the mixed path, each one untimed + `steps` timed solves."""
+    n = 16384
+    log(f"[bench] mixed16k: generating n={n}")
+    A0, lam_exact = workloads.make("mixed16k", n=n, device="cuda")
+    work = torch.empty_like(A0)
+
+    def one(precision="native"):
+        return _solve_timed(torch, cv, work, A0, bandwidth=64, block=512, precision=precision)
+
+    one()
+    ft = []
+    for i in range(steps):
+        dt, lam, V, _ = one()
+        ft.append(dt)
+    fp64_s = sum(ft) / len(ft)
+    res, orth = residual_orth(torch, A0, V, lam.double())
+    fp64_err = (lam.double() - lam_exact).abs().max().item() / lam_exact.abs().max().item()
+    del lam, V
+    log(f"[bench] mixed16k fp64: {fp64_s:.3f} s residual {res:.2e} orth {orth:.2e} eig err {fp64_err:.2e}")
+    try:
+        out = time_mixed(torch, cv, one, A0, lam_exact, 1, steps, fp64_s)
+    except cv.CuevError as e:
+        out = {"error": str(e), "fp64_seconds": round(fp64_s, 4)}
+    out["workload"] = "mixed16k: n=16384, b=64, k=512, lam geometric 1..1e-6, random signs, 10% in a 1e-9 cluster"
+    out["fp64"] = {"seconds": round(fp64_s, 4), "residual": res, "residual_bench_solver_form": res * n,
+                   "orthogonality": orth, "eig_err_vs_exact_over_norm2": fp64_err}
+    del work, A0
+    cv.cache_release()
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

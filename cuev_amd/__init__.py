@@ -4,8 +4,10 @@ Thin ctypes binding of the C ABI in include/cuev_c.h (libcuev_amd.so, built by
 ``python -m cuev_amd.build``). PyTorch supplies device memory and streams only.
 
 Layout: the library is column-major. A torch tensor ``t`` of shape (n, n) that
-is C-contiguous holds the column-major matrix ``t.T``; for symmetric inputs the
-two coincide. ``symm_eig_solve`` returns eigenvectors as the columns of ``V``.
+is C-contiguous holds the column-major matrix ``t.T``; one with strides (1, n)
+(``t.mT`` contiguous) holds ``t`` itself. ``symm_eig_solve`` reads the lower
+triangle of the tensor it is given, as ``torch.linalg.eigh`` does, and returns
+eigenvectors as the columns of ``V``.
 
 Mirrors the reference's entry point ``cuev::symm_eig_solve<T>(A, n, eval, evec,
 stream, timer)`` (include/cuev.h:31-33) and the stage-level launchers of
@@ -17,7 +19,7 @@ import os
 __all__ = [
     "CuevError", "SolveTimer", "Options", "lib", "lib_path", "symm_eig_solve",
     "eig_backward", "Handle", "gemm", "symm_lower", "syr2k_lower", "lapack_name",
-    "STATUS", "BUF", "profile_enable", "profile_read",
+    "STATUS", "BUF", "profile_enable", "profile_read", "cache_release",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -26,7 +28,7 @@ lib_path = os.path.join(_HERE, "libcuev_amd.so")
 STATUS = {
     0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "HIP_ERROR", 3: "ALLOC_FAILED",
     4: "LAPACK_ERROR", 5: "LAPACK_UNAVAILABLE", 6: "WATCHDOG", 7: "NOT_FINITE",
-    8: "NOT_SUPPORTED",
+    8: "NOT_SUPPORTED", 9: "NOT_CONVERGED",
 }
 BUF = {"Y": 0, "W": 1, "B": 2, "U": 3, "D": 4, "E": 5, "TRI": 6, "TAU": 7, "M": 8}
 PRECISION = {"native": 0, "mixed": 1}
@@ -101,6 +103,7 @@ def _declare(L):
         getattr(L, f"cuev_{pre}gemm").argtypes = [P, I, I, I64, I64, I64, sc, P, I64, P, I64, sc, P, I64]
         getattr(L, f"cuev_{pre}symm_lower").argtypes = [P, I64, I64, sc, P, I64, P, I64, sc, P, I64]
         getattr(L, f"cuev_{pre}syr2k_lower").argtypes = [P, I64, I64, sc, P, I64, P, I64, sc, P, I64]
+    L.cuev_cache_release.argtypes = []
     L.cuev_profile_enable.argtypes = [I]
     L.cuev_profile_read.argtypes = [ctypes.c_char_p, I, I]
     L.cuev_handle_create.argtypes = [ctypes.POINTER(P), I, I, I, I, P]
@@ -127,6 +130,11 @@ def profile_read(reset=True):
     buf = ctypes.create_string_buffer(max(need, 2) + 64)
     lib().cuev_profile_read(buf, len(buf), 1 if reset else 0)
     return json.loads(buf.value.decode() or "{}")
+
+
+def cache_release():
+    """Free the workspaces the library keeps between solves (all devices)."""
+    _check(lib().cuev_cache_release())
 
 
 def lapack_name():
@@ -160,11 +168,15 @@ def _ptr(t):
 def symm_eig_solve(A, *, overwrite=False, bandwidth=0, block=0, precision="native",
                    refine_sweeps=0, bt_mode="auto", dc_mode="auto", dc_leaf=0, refine_tol=0.0,
                    timer=False, stream=None):
-    """Eigen-decomposition of the symmetric (n, n) CUDA tensor ``A`` (lower triangle read).
+    """Eigen-decomposition of the symmetric (n, n) CUDA tensor ``A``.
 
-    Returns ``(eval, V)`` with ascending eigenvalues and eigenvectors in the columns
-    of ``V``; with ``timer=True`` also a dict of per-stage times. ``A`` is consumed
-    (as in the reference) only when ``overwrite=True``; otherwise a copy is solved.
+    Only the lower triangle of ``A`` (``A[i, j]`` with ``i >= j``) is read. Returns
+    ``(eval, V)`` with ascending eigenvalues and eigenvectors in the columns of
+    ``V``; with ``timer=True`` also a dict of per-stage times. ``A`` is consumed in
+    place (as in the reference) only when ``overwrite=True`` and ``A`` is stored
+    column-major (``A.mT.is_contiguous()``); otherwise a column-major copy is solved.
+    A status of NOT_CONVERGED (mixed form) raises ``CuevError`` after eval and V
+    were written.
     """
     torch = _torch()
     if A.dim() != 2 or A.shape[0] != A.shape[1]:
@@ -173,8 +185,8 @@ def symm_eig_solve(A, *, overwrite=False, bandwidth=0, block=0, precision="nativ
         raise ValueError("A must be a device tensor")
     n = A.shape[0]
     pre = _prefix(A.dtype)
-    work = A if (overwrite and A.is_contiguous()) else A.contiguous().clone()
-    # the contiguous buffer holds work.T column-major: for symmetric A these agree
+    # the library reads the buffer column-major: storage of A.mT in row-major order
+    work = A if (overwrite and A.mT.is_contiguous()) else A.mT.contiguous().mT
     ev = torch.empty(n, dtype=A.dtype, device=A.device)
     vec = torch.empty((n, n), dtype=A.dtype, device=A.device)
     t = SolveTimer()

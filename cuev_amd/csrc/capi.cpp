@@ -193,6 +193,14 @@ template <typename T> int stage_sbr_back(cuev_handle_t h, const T *Y, const T *W
 
 extern "C" {
 
+int cuev_cache_release(void) {
+    detail::clear_error();
+    context_release_all<float>();
+    context_release_all<double>();
+    mixed_release_all();
+    return CUEV_STATUS_SUCCESS;
+}
+
 const char *cuev_status_string(int status) {
     switch (status) {
     case CUEV_STATUS_SUCCESS: return "success";
@@ -204,6 +212,7 @@ const char *cuev_status_string(int status) {
     case CUEV_STATUS_WATCHDOG: return "persistent-kernel watchdog expired";
     case CUEV_STATUS_NOT_FINITE: return "non-finite values";
     case CUEV_STATUS_NOT_SUPPORTED: return "not supported";
+    case CUEV_STATUS_NOT_CONVERGED: return "refinement not converged";
     default: return "unknown status";
     }
 }

@@ -158,6 +158,29 @@ template <typename T> void context_release(Context<T> *ctx) {
     if (victim) context_destroy(victim);
 }
 
+template <typename T> void context_release_all() {
+    auto &c = cache<T>();
+    Context<T> *victims[64] = {};
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (int i = 0; i < 64; ++i) {
+            victims[i] = c.idle[i];
+            c.idle[i] = nullptr;
+        }
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (Context<T> *v : victims)
+        if (v) {
+            (void)hipSetDevice(v->ws.device);
+            (void)hipDeviceSynchronize();
+            context_destroy(v);
+        }
+    (void)hipSetDevice(cur);
+}
+
+template void context_release_all<float>();
+template void context_release_all<double>();
 template int context_create<float>(Context<float> *&, int, int, int, hipStream_t);
 template int context_create<double>(Context<double> *&, int, int, int, hipStream_t);
 template void context_destroy<float>(Context<float> *);

@@ -46,5 +46,7 @@ template <typename T> void context_destroy(Context<T> *ctx);
 // per (device, precision); a busy or mismatched entry is bypassed.
 template <typename T> int context_acquire(Context<T> *&ctx, int n, int b, int k, hipStream_t s);
 template <typename T> void context_release(Context<T> *ctx);
+// Free every idle cached context of this precision (all devices).
+template <typename T> void context_release_all();
 
 } // namespace cuev

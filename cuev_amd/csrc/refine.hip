@@ -289,6 +289,30 @@ int mixed_acquire(MixedBuffers *&m, int n) {
     return mixed_alloc(m, n);
 }
 
+} // namespace
+
+void mixed_release_all() {
+    MixedBuffers *victims[64] = {};
+    {
+        std::lock_guard<std::mutex> lk(g_mix_mu);
+        for (int i = 0; i < 64; ++i) {
+            victims[i] = g_mix_idle[i];
+            g_mix_idle[i] = nullptr;
+        }
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (MixedBuffers *v : victims)
+        if (v) {
+            (void)hipSetDevice(v->device);
+            (void)hipDeviceSynchronize();
+            mixed_free(v);
+        }
+    (void)hipSetDevice(cur);
+}
+
+namespace {
+
 void mixed_release(MixedBuffers *m) {
     MixedBuffers *victim = nullptr;
     {
@@ -637,6 +661,13 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         timer->refine_resid = (float)res;
         timer->refine_orth = (float)orth;
     }
+    // the sweep limit was reached above the tolerance: the outputs hold the last
+    // iterate, and the caller is told it is not FP64-accurate
+    if (!(res <= tol && orth <= tol))
+        CUEV_FAIL(CUEV_STATUS_NOT_CONVERGED,
+                  "mixed: %d refinement sweeps left residual %.3e and orthogonality %.3e above "
+                  "refine_tol %.3e",
+                  oa_sweeps, res, orth, tol);
     return CUEV_STATUS_SUCCESS;
 }
 
@@ -653,7 +684,7 @@ int solve_mixed(double *A, int n, double *eval, double *evec, hipStream_t s, cue
         return st;
     }
     st = run_mixed(*cf, *mb, A, n, eval, evec, timer, opt);
-    if (st == CUEV_STATUS_SUCCESS) {
+    if (st == CUEV_STATUS_SUCCESS || st == CUEV_STATUS_NOT_CONVERGED) {
         context_release<float>(cf);
         mixed_release(mb);
     } else {

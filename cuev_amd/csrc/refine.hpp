@@ -6,6 +6,9 @@
 
 namespace cuev {
 
+// Free the idle cached FP64 refinement buffers of every device.
+void mixed_release_all();
+
 int solve_mixed(double *A, int n, double *eval, double *evec, hipStream_t s, cuev_solve_timer *timer,
                 const cuev_options &opt, int b, int k);
 

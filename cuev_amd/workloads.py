@@ -56,6 +56,33 @@ def sandwich(torch, Q, lam):
     return (A + A.T) * 0.5
 
 
+def residual_orth(A0, V, lam, cols=2048):
+    """(||A V - V diag(lam)||_F / (n ||A||_F), ||V^T V - I||_F / n) in float64 on A0's
+    device, accumulated over column blocks of V so that the check needs O(n * cols)
+    extra memory (at n = 49152 the solver's cached workspace and three n x n matrices
+    already fill most of HBM). Multiply the first by n for the form
+    ||A V - V diag(lam)||_F / ||A||_F."""
+    import torch
+    n = A0.shape[0]
+    A0 = A0.double()
+    V = V.double()
+    lam = lam.double()
+    an = torch.linalg.matrix_norm(A0).item()
+    an = an if an > 0 else 1.0
+    r2 = o2 = 0.0
+    for c0 in range(0, n, cols):
+        c1 = min(n, c0 + cols)
+        Vc = V[:, c0:c1]
+        R = A0 @ Vc
+        R.sub_(Vc * lam[c0:c1])
+        r2 += torch.linalg.matrix_norm(R).item() ** 2
+        G = Vc.T @ V
+        G[:, c0:c1].diagonal().sub_(1.0)
+        o2 += torch.linalg.matrix_norm(G).item() ** 2
+        del R, G
+    return r2 ** 0.5 / (n * an), o2 ** 0.5 / n
+
+
 def make(name, n=None, device="cuda", dtype=None):
     """Return (A, lam_exact or None) as torch tensors on `device` (float64 unless dtype)."""
     import torch

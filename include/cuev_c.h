@@ -53,7 +53,9 @@ typedef enum cuev_status_t {
     CUEV_STATUS_LAPACK_UNAVAILABLE = 5,/* no host LAPACK with the needed routine found */
     CUEV_STATUS_WATCHDOG = 6,          /* a persistent kernel's bounded wait expired */
     CUEV_STATUS_NOT_FINITE = 7,        /* NaN/Inf reached the tridiagonal solver */
-    CUEV_STATUS_NOT_SUPPORTED = 8      /* option combination not implemented */
+    CUEV_STATUS_NOT_SUPPORTED = 8,     /* option combination not implemented */
+    CUEV_STATUS_NOT_CONVERGED = 9      /* mixed form: the refinement reached its sweep limit
+                                          above refine_tol; eval/evec hold the last iterate */
 } cuev_status_t;
 
 /* Human-readable name of a status code. */
@@ -121,6 +123,13 @@ typedef struct cuev_options {
     double refine_tol;   /* mixed form: stop once the FP64 relative residual
                             ||AV - V diag(lambda)||_F / (n ||A||_F) is below this; 0 selects 2e-14 */
 } cuev_options;
+
+/* The drop-in entry points keep the workspace of the last problem size per
+ * device and precision for the next call (the reference allocates and frees it
+ * on every call, src/cuda/handle.cu:73 and solver.cu:72). This frees every idle
+ * cached workspace now; it must not race with a solve in progress. No reference
+ * counterpart. */
+int cuev_cache_release(void);
 
 /* Fill *opt with the defaults listed above. */
 void cuev_options_init(cuev_options *opt);

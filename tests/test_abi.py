@@ -40,7 +40,7 @@ def test_all_declared_symbols_exported(L):
 
 
 def test_status_strings(L):
-    for code in range(9):
+    for code in range(10):
         assert L.cuev_status_string(code)
     assert b"watchdog" in L.cuev_status_string(6)
 

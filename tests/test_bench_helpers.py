@@ -22,3 +22,18 @@ def test_residual_orth_blocked_equals_whole():
     orth_ref = np.linalg.norm(V.T @ V - np.eye(n)) / n
     assert abs(res - res_ref) <= 1e-12 * res_ref
     assert abs(orth - orth_ref) <= 1e-12 * orth_ref
+
+
+def test_cpu_baseline_same_matrix():
+    """The CPU baseline times dsyevd on the matrix it is given and reports its accuracy
+    against the exact spectrum when one is known."""
+    rng = np.random.default_rng(4)
+    n = 120
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.sort(rng.uniform(-1, 1, n))
+    A = (Q * lam) @ Q.T
+    A = (A + A.T) / 2
+    out = bench.cpu_baseline_same(A.copy(), lam)
+    assert out["value"] > 0 and out["kind"] == "port" and out["cores"] >= 1
+    assert f"n={n}" in out["sample"] and "not scaled" in out["sample"]
+    assert out["eig_err_vs_exact_over_norm2"] < 1e-13

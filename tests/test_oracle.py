@@ -180,13 +180,13 @@ def test_dbbr_ragged_last_block_stays_in_bounds(n, b, k):
 
 
 def _bc_back_blocked(U, M, b, ng=16):
-    """The application order of the device bc_back kernel, in numpy: passes of 256/b hop
-    indices; inside a pass, groups of ng sweeps from the last down; inside a group, its
+    """The application order of the device bc_back kernel, in numpy: passes of 128/b hop
+    indices (backtransform.hip: G = 128 / B); inside a pass, groups of ng sweeps from the last down; inside a group, its
     hops upward, each as one block reflector I - V T V^T with
     T = (I/2 + striu(V^T V))^-1 (reflectors I - 2 w w^T)."""
     n = M.shape[0]
     M = M.copy()
-    G, last, h0 = 256 // b, n - 3, 0
+    G, last, h0 = 128 // b, n - 3, 0
     rows_per_block = 16 * (b // 16 + 1)
     while min(last, n - 2 - h0 * b) >= 0:
         smax = min(last, n - 2 - h0 * b)
