@@ -19,7 +19,8 @@ import os
 __all__ = [
     "CuevError", "SolveTimer", "Options", "lib", "lib_path", "symm_eig_solve",
     "eig_backward", "Handle", "gemm", "symm_lower", "syr2k_lower", "lapack_name",
-    "STATUS", "BUF", "profile_enable", "profile_read", "cache_release",
+    "STATUS", "BUF", "profile_enable", "profile_read", "cache_release", "symm_eig_values",
+    "tridi_values", "symm_eig_batched",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -89,6 +90,10 @@ def _declare(L):
     for pre in ("s", "d"):
         getattr(L, f"cuev_{pre}symm_eig_solve").argtypes = [P, I, P, P, P, P]
         getattr(L, f"cuev_{pre}symm_eig_solve_ex").argtypes = [P, I, P, P, P, P, ctypes.POINTER(Options)]
+        getattr(L, f"cuev_{pre}symm_eig_values").argtypes = [P, I, P, P, P]
+        getattr(L, f"cuev_{pre}symm_eig_values_ex").argtypes = [P, I, I, I, P, P, P, ctypes.POINTER(Options)]
+        getattr(L, f"cuev_{pre}tridi_values").argtypes = [P, P, I, I, I, P, P]
+        getattr(L, f"cuev_{pre}symm_eig_batched").argtypes = [P, I, I, P, P, P, ctypes.POINTER(I)]
         getattr(L, f"cuev_{pre}symm_eig_backward").argtypes = [P, P, P, P, I, P, D, ctypes.POINTER(I), P]
         getattr(L, f"cuev_{pre}_dbbr_pack").argtypes = [P, P, P, I, I]
         getattr(L, f"cuev_{pre}_dbbr_panel_qr").argtypes = [P, P, P, I, I]
@@ -205,6 +210,64 @@ def symm_eig_solve(A, *, overwrite=False, bandwidth=0, block=0, precision="nativ
     if timer:
         return ev, V, t.as_dict()
     return ev, V
+
+
+def symm_eig_values(A, il=0, iu=None, *, overwrite=False, bandwidth=0, block=0, timer=False,
+                    stream=None):
+    """Eigenvalues il..iu (0-based, inclusive, ascending) of the symmetric CUDA tensor
+    ``A`` (lower triangle read), without eigenvectors: band reduction, bulge chasing
+    and Sturm multisection on the GPU. Returns the eigenvalues (and the stage times
+    with ``timer=True``). ``A`` is consumed in place only with ``overwrite=True`` and
+    column-major storage, as in ``symm_eig_solve``."""
+    torch = _torch()
+    if A.dim() != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("A must be square")
+    if not A.is_cuda:
+        raise ValueError("A must be a device tensor")
+    n = A.shape[0]
+    iu = n - 1 if iu is None else int(iu)
+    pre = _prefix(A.dtype)
+    work = A if (overwrite and A.mT.is_contiguous()) else A.mT.contiguous().mT
+    ev = torch.empty(max(iu - int(il) + 1, 0), dtype=A.dtype, device=A.device)
+    t = SolveTimer()
+    o = Options()
+    lib().cuev_options_init(ctypes.byref(o))
+    o.bandwidth, o.block = int(bandwidth), int(block)
+    fn = getattr(lib(), f"cuev_{pre}symm_eig_values_ex")
+    _check(fn(_ptr(work), n, int(il), iu, _ptr(ev), _stream_ptr(stream), ctypes.byref(t), ctypes.byref(o)))
+    return (ev, t.as_dict()) if timer else ev
+
+
+def symm_eig_batched(A, stream=None):
+    """Eigen-decompositions of a batch of symmetric matrices, A of shape (batch, n, n)
+    with n <= 256 (lower triangle of each A[b] read, as torch.linalg.eigh does).
+    Returns (eval (batch, n) ascending, V (batch, n, n) with eigenvectors as columns)."""
+    torch = _torch()
+    if A.dim() != 3 or A.shape[1] != A.shape[2]:
+        raise ValueError("A must have shape (batch, n, n)")
+    batch, n = A.shape[0], A.shape[1]
+    pre = _prefix(A.dtype)
+    work = A.mT.contiguous()  # each matrix column-major (the library's layout)
+    ev = torch.empty((batch, n), dtype=A.dtype, device=A.device)
+    vec = torch.empty((batch, n, n), dtype=A.dtype, device=A.device)
+    nbad = ctypes.c_int(0)
+    _check(getattr(lib(), f"cuev_{pre}symm_eig_batched")(_ptr(work), n, batch, _ptr(ev), _ptr(vec),
+                                                         _stream_ptr(stream), ctypes.byref(nbad)))
+    return ev, vec.mT
+
+
+def tridi_values(d, e, il=0, iu=None, stream=None):
+    """Eigenvalues il..iu of tridiag(e, d, e) (device tensors; e has n-1 or n entries)."""
+    torch = _torch()
+    n = d.shape[0]
+    iu = n - 1 if iu is None else int(iu)
+    pre = _prefix(d.dtype)
+    d = d.contiguous()
+    e = e.contiguous()
+    w = torch.empty(max(iu - int(il) + 1, 0), dtype=d.dtype, device=d.device)
+    _check(getattr(lib(), f"cuev_{pre}tridi_values")(_ptr(d), _ptr(e), n, int(il), iu, _ptr(w),
+                                                     _stream_ptr(stream)))
+    return w
 
 
 def eig_backward(V, lam, Vbar=None, lambar=None, gap_tol=0.0, stream=None):

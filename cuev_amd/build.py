@@ -69,7 +69,7 @@ def build(jobs=None, debug=False, verbose=False):
     tmp = LIB + ".tmp"
     rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + [
-        f"-L{rocm_lib}", f"-Wl,-rpath,{rocm_lib}", "-lrocblas", "-ldl", "-lm", "-lpthread"]
+        f"-L{rocm_lib}", f"-Wl,-rpath,{rocm_lib}", "-ldl", "-lm", "-lpthread"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         print(p.stderr, file=sys.stderr)

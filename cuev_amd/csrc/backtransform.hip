@@ -826,10 +826,6 @@ int sbr_back_t(Workspace<T> &ws, hipStream_t s, const T *Y, const T *W, T *M, i6
     const i64 ld = ws.ld;
     if (ldm < n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "sbr_back_t: ldm < n");
     Scratch gs{ws.gscratch, ws.gscratch_bytes};
-    // the block products are plain rank-k shapes: rocBLAS runs them at 64-69 TF/s
-    // against 50-57 for the tile engine (n = 16384: 342 -> 270 ms beside the chase;
-    // n = 49152: 4.2 -> 3.7 s, where they outlast the chase)
-    gs.vendor = 1;
     // Q_s^T = ... (I - Yhat_1 What_1^T)(I - Yhat_0 What_0^T): blocks first to last,
     // M -= Yhat (What^T M) with Yhat the block's Y columns, zero above each panel's
     // rows (copied into ws.Yh, since Y there holds nothing defined)

@@ -11,9 +11,16 @@
 //
 // Products: C = V^T Vbar, then K = diag(lambar) + F o skew(C) in one fused
 // element-wise pass over pairs (i < j) that writes K[i,j] and K[j,i] in place,
-// then T = V K and Abar = T V^T (rocBLAS for large n, the MFMA tile engine
-// below that). Only V and lambda are needed from the forward pass.
+// then T = V K and Abar = T V^T, all on the MFMA tile engine. Only V and lambda
+// are needed from the forward pass.
+//
+// Work buffers (C and T, n x n each, and the masked-pair counter):
+//  * cuev_{s,d}symm_eig_backward keeps them per device between calls (allocated
+//    on the first call of a size, grown when a larger n comes; a call that finds
+//    the cached buffer in use by another call allocates its own).
 #include "backward.hpp"
+
+#include <mutex>
 
 #include "gemm.hpp"
 #include "util.hpp"
@@ -45,61 +52,114 @@ __global__ __launch_bounds__(256) void eig_bwd_scale_kernel(T *C, i64 ldc, const
     C[j + (i64)i * ldc] = v;
 }
 
+// The backward pass on caller-provided work buffers (C, Tm: n x n; dmask: one int).
+template <typename T>
+int backward_core(const T *V, const T *lam, const T *Vbar, const T *lambar, int n, T *Abar, double gap_tol,
+                  int *n_masked_host, hipStream_t s, T *C, T *Tm, int *dmask, Scratch gs, bool sync) {
+    const size_t nn = (size_t)n * n;
+    CUEV_TRY(fill_int(s, dmask, 1, 0));
+    if (Vbar) {
+        CUEV_TRY(gemm<T>(s, 1, 0, n, n, n, T(1), V, n, Vbar, n, T(0), C, n, gs));
+    } else {
+        CUEV_TRY(fill2d<T>(s, C, n, n, n, T(0), T(0)));
+    }
+    hipLaunchKernelGGL(eig_bwd_scale_kernel<T>, dim3((unsigned)ceil_div((i64)nn, 256)), dim3(256), 0, s, C,
+                       (i64)n, lam, lambar, n, (T)gap_tol, dmask);
+    CUEV_LAUNCH_CHECK();
+    CUEV_TRY(gemm<T>(s, 0, 0, n, n, n, T(1), V, n, C, n, T(0), Tm, n, gs));
+    CUEV_TRY(gemm<T>(s, 0, 1, n, n, n, T(1), Tm, n, V, n, T(0), Abar, n, gs));
+    if (!n_masked_host && !sync) return CUEV_STATUS_SUCCESS;
+    int hm = 0;
+    CUEV_HIP(hipMemcpyAsync(&hm, dmask, sizeof(int), hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    if (n_masked_host) *n_masked_host = hm;
+    return CUEV_STATUS_SUCCESS;
+}
+
+int check_args(const void *V, const void *lam, const void *Abar, int n) {
+    if (n < 1 || !V || !lam || !Abar)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "eig_backward: n=%d V=%p lambda=%p Abar=%p", n, V, lam, Abar);
+    return CUEV_STATUS_SUCCESS;
+}
+
+// ---- cached work buffer of the allocating entry point (one idle buffer per device)
+struct BwdBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+};
+std::mutex g_bwd_mu;
+BwdBuf g_bwd_idle[64];
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
 } // namespace
 
 template <typename T>
 int eig_backward(const T *V, const T *lam, const T *Vbar, const T *lambar, int n, T *Abar,
                  double gap_tol, int *n_masked_host, hipStream_t s) {
-    if (n < 1 || !V || !lam || !Abar)
-        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "eig_backward: n=%d V=%p lambda=%p Abar=%p", n,
-                  (const void *)V, (const void *)lam, (void *)Abar);
-    const size_t nn = (size_t)n * n;
-    T *C = nullptr, *Tm = nullptr;
-    int *dmask = nullptr;
-    void *scr = nullptr;
-    const size_t scr_bytes = gemm_scratch_bytes(n, n, sizeof(T));
-    auto cleanup = [&]() {
-        if (C) (void)hipFree(C);
-        if (Tm) (void)hipFree(Tm);
-        if (dmask) (void)hipFree(dmask);
-        if (scr) (void)hipFree(scr);
-    };
-    if (hipMalloc(&C, nn * sizeof(T)) != hipSuccess || hipMalloc(&Tm, nn * sizeof(T)) != hipSuccess ||
-        hipMalloc(&dmask, sizeof(int)) != hipSuccess || hipMalloc(&scr, scr_bytes) != hipSuccess) {
-        cleanup();
-        CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "eig_backward: device allocation failed (n=%d)", n);
+    CUEV_TRY(check_args(V, lam, Abar, n));
+    const size_t mat = align256((size_t)n * n * sizeof(T));
+    const size_t gsb = gemm_scratch_bytes(n, n, sizeof(T));
+    const size_t need = 2 * mat + 256 + gsb;
+    int dev = 0;
+    CUEV_HIP(hipGetDevice(&dev));
+    BwdBuf buf;
+    {
+        std::lock_guard<std::mutex> lk(g_bwd_mu);
+        BwdBuf &slot = g_bwd_idle[dev & 63];
+        if (slot.p && slot.device == dev && slot.bytes >= need) {
+            buf = slot;
+            slot = BwdBuf{};
+        }
     }
-    Scratch gs{scr, scr_bytes};
-    gs.vendor = 1; // three plain n^3 products with the GPU to themselves: rocBLAS when large
-    int st = CUEV_STATUS_SUCCESS;
-    do {
-        if ((st = fill_int(s, dmask, 1, 0))) break;
-        if (Vbar) {
-            if ((st = gemm<T>(s, 1, 0, n, n, n, T(1), V, n, Vbar, n, T(0), C, n, gs))) break;
-        } else {
-            if ((st = fill2d<T>(s, C, n, n, n, T(0), T(0)))) break;
-        }
-        hipLaunchKernelGGL(eig_bwd_scale_kernel<T>, dim3((unsigned)ceil_div((i64)nn, 256)),
-                           dim3(256), 0, s, C, (i64)n, lam, lambar, n, (T)gap_tol, dmask);
-        if (hipGetLastError() != hipSuccess) {
-            detail::set_error("eig_backward: scale kernel launch failed");
-            st = CUEV_STATUS_HIP_ERROR;
-            break;
-        }
-        if ((st = gemm<T>(s, 0, 0, n, n, n, T(1), V, n, C, n, T(0), Tm, n, gs))) break;
-        if ((st = gemm<T>(s, 0, 1, n, n, n, T(1), Tm, n, V, n, T(0), Abar, n, gs))) break;
-        int hm = 0;
-        if (hipMemcpyAsync(&hm, dmask, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            detail::set_error("eig_backward: synchronisation failed");
-            st = CUEV_STATUS_HIP_ERROR;
-            break;
-        }
-        if (n_masked_host) *n_masked_host = hm;
-    } while (0);
+    if (!buf.p) {
+        buf.device = dev;
+        buf.bytes = need;
+        if (hipMalloc(&buf.p, need) != hipSuccess)
+            CUEV_FAIL(CUEV_STATUS_ALLOC_FAILED, "eig_backward: device allocation of %zu bytes failed (n=%d)", need, n);
+    }
+    auto *base = static_cast<unsigned char *>(buf.p);
+    T *C = reinterpret_cast<T *>(base);
+    T *Tm = reinterpret_cast<T *>(base + mat);
+    int *dmask = reinterpret_cast<int *>(base + 2 * mat);
+    Scratch gs{base + 2 * mat + 256, gsb};
+    const int st =
+        backward_core<T>(V, lam, Vbar, lambar, n, Abar, gap_tol, n_masked_host, s, C, Tm, dmask, gs, true);
     (void)hipStreamSynchronize(s);
-    cleanup();
+    BwdBuf victim;
+    {
+        std::lock_guard<std::mutex> lk(g_bwd_mu);
+        BwdBuf &slot = g_bwd_idle[dev & 63];
+        if (slot.p && slot.bytes >= buf.bytes) {
+            victim = buf; // keep the larger of the two
+        } else {
+            victim = slot;
+            slot = buf;
+        }
+    }
+    if (victim.p) (void)hipFree(victim.p);
     return st;
+}
+
+void backward_release_all() {
+    BwdBuf victims[64];
+    {
+        std::lock_guard<std::mutex> lk(g_bwd_mu);
+        for (int i = 0; i < 64; ++i) {
+            victims[i] = g_bwd_idle[i];
+            g_bwd_idle[i] = BwdBuf{};
+        }
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (BwdBuf &v : victims)
+        if (v.p) {
+            (void)hipSetDevice(v.device);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(v.p);
+        }
+    (void)hipSetDevice(cur);
 }
 
 template int eig_backward<float>(const float *, const float *, const float *, const float *, int,

@@ -8,5 +8,7 @@ namespace cuev {
 template <typename T>
 int eig_backward(const T *V, const T *lam, const T *Vbar, const T *lambar, int n, T *Abar,
                  double gap_tol, int *n_masked, hipStream_t s);
+// Free the cached work buffers of eig_backward (all devices).
+void backward_release_all();
 
 } // namespace cuev

@@ -3,6 +3,7 @@
 
 #include "backtransform.hpp"
 #include "backward.hpp"
+#include "batched.hpp"
 #include "bulge.hpp"
 #include "context.hpp"
 #include "dbbr.hpp"
@@ -10,6 +11,7 @@
 #include "lapack.hpp"
 #include "refine.hpp"
 #include "solver.hpp"
+#include "tridi_values.hpp"
 #include "util.hpp"
 
 using namespace cuev;
@@ -75,6 +77,41 @@ int solve_entry(T *A, int n, T *eval, T *evec, hipStream_t s, cuev_solve_timer *
         context_destroy<T>(ctx);
     }
     return st;
+}
+
+template <typename T>
+int values_entry(T *A, int n, int il, int iu, T *eval, hipStream_t s, cuev_solve_timer *timer,
+                 const cuev_options *opt) {
+    detail::clear_error();
+    cuev_options o;
+    options_defaults(o);
+    if (opt) o = *opt;
+    if (n < 1) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_values: n must be >= 1 (got %d)", n);
+    if (!A || !eval) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_values: null device pointer");
+    if (il < 0 || iu >= n || il > iu)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "symm_eig_values: index range [%d, %d] outside [0, %d)", il, iu,
+                  n);
+    if (o.precision != CUEV_PRECISION_NATIVE)
+        CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "symm_eig_values: the eigenvalue-only path runs in the API's precision");
+    int b = o.bandwidth, k = o.block;
+    CUEV_TRY(resolve_blocking(n, b, k));
+    if (k > n) k = (int)round_up(n, b);
+    Context<T> *ctx = nullptr;
+    CUEV_TRY(context_acquire<T>(ctx, n, b, k, s));
+    const int st = solve_values<T>(*ctx, A, il, iu, eval, timer);
+    if (st == CUEV_STATUS_SUCCESS) {
+        context_release<T>(ctx);
+    } else {
+        (void)hipStreamSynchronize(s);
+        context_destroy<T>(ctx);
+    }
+    return st;
+}
+
+template <typename T>
+int tridi_values_entry(const T *d, const T *e, int n, int il, int iu, T *w, hipStream_t s) {
+    detail::clear_error();
+    return tridi_values<T>(s, d, e, n, il, iu, w);
 }
 
 template <typename T>
@@ -193,11 +230,45 @@ template <typename T> int stage_sbr_back(cuev_handle_t h, const T *Y, const T *W
 
 extern "C" {
 
+int cuev_ssymm_eig_values(float *A, int n, float *eval, hipStream_t stream, cuev_solve_timer *timer) {
+    return values_entry<float>(A, n, 0, n - 1, eval, stream, timer, nullptr);
+}
+int cuev_dsymm_eig_values(double *A, int n, double *eval, hipStream_t stream, cuev_solve_timer *timer) {
+    return values_entry<double>(A, n, 0, n - 1, eval, stream, timer, nullptr);
+}
+int cuev_ssymm_eig_values_ex(float *A, int n, int il, int iu, float *eval, hipStream_t stream,
+                             cuev_solve_timer *timer, const cuev_options *opt) {
+    return values_entry<float>(A, n, il, iu, eval, stream, timer, opt);
+}
+int cuev_dsymm_eig_values_ex(double *A, int n, int il, int iu, double *eval, hipStream_t stream,
+                             cuev_solve_timer *timer, const cuev_options *opt) {
+    return values_entry<double>(A, n, il, iu, eval, stream, timer, opt);
+}
+int cuev_stridi_values(const float *d, const float *e, int n, int il, int iu, float *w, hipStream_t stream) {
+    return tridi_values_entry<float>(d, e, n, il, iu, w, stream);
+}
+int cuev_dtridi_values(const double *d, const double *e, int n, int il, int iu, double *w,
+                       hipStream_t stream) {
+    return tridi_values_entry<double>(d, e, n, il, iu, w, stream);
+}
+
+int cuev_ssymm_eig_batched(float *A, int n, int batch, float *eval, float *evec, hipStream_t stream,
+                           int *n_not_converged) {
+    detail::clear_error();
+    return eig_batched<float>(stream, A, n, batch, eval, evec, n_not_converged);
+}
+int cuev_dsymm_eig_batched(double *A, int n, int batch, double *eval, double *evec, hipStream_t stream,
+                           int *n_not_converged) {
+    detail::clear_error();
+    return eig_batched<double>(stream, A, n, batch, eval, evec, n_not_converged);
+}
+
 int cuev_cache_release(void) {
     detail::clear_error();
     context_release_all<float>();
     context_release_all<double>();
     mixed_release_all();
+    backward_release_all();
     return CUEV_STATUS_SUCCESS;
 }
 

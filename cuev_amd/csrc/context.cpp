@@ -91,12 +91,10 @@ template <typename T> void context_destroy(Context<T> *ctx) {
     if (ctx->bt_stream) (void)hipStreamSynchronize(ctx->bt_stream);
     for (auto &e : ctx->xev)
         if (e) (void)hipEventDestroy(e);
-    if (ctx->bt_stream) gemm_vendor_release(ctx->bt_stream);
     if (ctx->bt_stream) (void)hipStreamDestroy(ctx->bt_stream);
     if (ctx->dc_stream) (void)hipStreamSynchronize(ctx->dc_stream);
     dc_state_destroy(ctx->dc);
     if (ctx->dc_pool) (void)hipFree(ctx->dc_pool);
-    if (ctx->dc_stream) gemm_vendor_release(ctx->dc_stream);
     if (ctx->dc_stream) (void)hipStreamDestroy(ctx->dc_stream);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

@@ -583,8 +583,6 @@ template <typename T>
 int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q, i64 ldq, T *eval,
              T *Qc, T *S, T *Qn, Scratch gs, int leaf, hipEvent_t gate, int gate_from) {
     if (n <= 0) return CUEV_STATUS_SUCCESS;
-    // large merges that have the GPU to themselves use the vendor library's tiles
-    gs.vendor = gs.small_tiles ? 0 : 1;
     if (n > st.n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "dc: state sized for %d, got n=%d", st.n, n);
     if (ldq < n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "dc: ldq < n");
     if (leaf <= 0) leaf = 64;
@@ -695,7 +693,6 @@ int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q,
             CUEV_HIP(hipStreamWaitEvent(s, gate, 0));
             gate = nullptr;
             gs.small_tiles = 0;
-            gs.vendor = 1;
         }
         // z and the current eigenvalues to the host
         CUEV_HIP(hipMemcpyAsync(st.d_up + L.md0, md0, sizeof(MergeDesc) * nm, hipMemcpyHostToDevice, s));

@@ -1,9 +1,28 @@
 // gemm.hip — MFMA tile engine for gfx950 (FP64 and FP32).
 //
-// Block tile BM x BN x 16, 256 threads = 4 wave64s arranged 2 x 2, each wave
-// owning a (BM/2) x (BN/2) sub-tile of 16 x 16 MFMA fragments. Operands are
-// staged global -> registers -> LDS (two LDS buffers, one barrier per k-step,
-// the next k-tile's global loads in flight while the current one computes).
+// Every dense product of the solver runs here: plain products (N/T operands),
+// symmetric-from-lower A (symm), K-concatenation of two sources (the band
+// reduction's corrections, green update and syr2k as one product each), lower-only
+// output, split-K.
+//
+// Block tile BM x BN x BK with WM x WN wave64s, each owning a (BM/WM) x (BN/WN)
+// sub-tile of 16 x 16 MFMA fragments (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32,
+// FP32 accumulation for FP32). BK is 16 elements (a K-concatenation split must be a
+// multiple of it).
+//
+// Operand staging: global -> registers -> LDS, double-buffered in both, one barrier
+// per k-tile; the next k-tile's global loads are in flight while the current one
+// computes. Global loads are 16 bytes wide (2 doubles / 4 floats) wherever the tile
+// lies inside the operand and the operand is 16-byte aligned with ld a multiple of
+// the vector width; edge tiles, symmetric tiles straddling the diagonal and
+// unaligned operands take a per-element path with bounds checks into the same LDS
+// layout. In LDS both operands are stored row-major with K contiguous ([m][k] and
+// [n][k]), rows padded by 16 bytes so that the fragment reads of 16 consecutive
+// rows hit distinct banks. A fragment read is one 16-byte ds_read per lane: lane
+// (i = l%16, j = l/16) reads row i at k = E*j .. E*j+E-1 (E = 2 doubles / 4
+// floats), which feeds E MFMAs at once. Each MFMA then contracts lane group j's
+// k = E*j + e; A and B use the same k assignment, so the sum over the E MFMAs and
+// the 4 lane groups covers every k of the group exactly once.
 //
 // The MFMA is issued with the B fragment in the instruction's A slot and the A
 // fragment in its B slot, so the accumulator's 16 consecutive lanes hold 16
@@ -19,29 +38,25 @@
 // tiles, then grouped 8 tile-rows at a time, keeping operand panels shared by
 // neighbouring tiles inside one XCD's L2.
 //
-// Tall-skinny products (the DBBR panel products with K = n - j - b, the
-// sbr_back Y^T M) split K over blockIdx.y into a scratch buffer, summed in a
+// Tall-skinny products (the band reduction's panel products with K = n - j - b,
+// sbr_back's Y^T M) split K over blockIdx.y into a scratch buffer, summed in a
 // fixed order by a second kernel, so results are deterministic.
 #include "gemm.hpp"
 
 #include <algorithm>
 #include <cmath>
-#include <mutex>
-#include <utility>
-#include <vector>
-
-#include <rocblas/rocblas.h>
 
 namespace cuev {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int BK = 16;
 constexpr int kGroupM = 8;
 
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
+    typedef double vec __attribute__((ext_vector_type(2)));
+    static constexpr int E = 2;   // elements per 16-byte vector
+    static constexpr int BK = 16; // k per tile
     __device__ static __forceinline__ v4 mma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -49,6 +64,9 @@ template <> struct Mfma<double> {
 };
 template <> struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
+    typedef float vec __attribute__((ext_vector_type(4)));
+    static constexpr int E = 4;
+    static constexpr int BK = 16;
     __device__ static __forceinline__ v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -58,10 +76,8 @@ template <> struct Mfma<float> {
 template <typename T> struct KParams {
     const T *a0, *a1;
     i64 lda0, lda1, asplit;
-    int atrans, asym;
     const T *b0, *b1;
     i64 ldb0, ldb1, bsplit;
-    int btrans;
     T *C;
     i64 ldc;
     T *part; // split-K partials: part[s*M*N + m + n*M]
@@ -69,6 +85,7 @@ template <typename T> struct KParams {
     T alpha, beta;
     int lower;
     int tiles_m, tiles_n;
+    int avec, bvec; // 16-byte loads allowed (alignment of both sources and their ld)
 };
 
 // Map the (remapped) linear tile id to (tile_m, tile_n).
@@ -93,121 +110,118 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
 }
 
 // Operand modes, fixed at compile time so that each kernel carries only the
-// loader it needs (register pressure decides how many waves share a SIMD).
+// loaders it needs (register pressure decides how many waves share a SIMD).
 enum AMode { A_N = 0, A_T = 1, A_SYM = 2, A_CAT = 3 }; // A_CAT: two m-contiguous sources
 enum BMode { B_N = 0, B_T = 1, B_CATT = 2 };           // B_CATT: two n-contiguous sources
 
-// A tile (BM x BK at rows m0, k in [k0, k0+BK)) into registers, in the order
-// store_a expects. Pattern 0: m-contiguous (element (m,k) at p[m + k*ld]);
-// pattern 1: k-contiguous (at p[k + m*ld]); pattern 2: symmetric straddling the
-// diagonal (lower triangle stored).
-template <typename T, int BM, int NA>
-__device__ __forceinline__ void load_a(T (&ra)[NA], int pat, const T *pa, i64 lda, i64 m0, i64 k0,
-                                       i64 kend, i64 M, int tid) {
-    if (pat == 1) {
-        const int kk = tid % BK;
-        const i64 k = k0 + kk;
-        const bool kok = k < kend;
+// Source layouts of one R x BK operand tile (rows r = m or n of the product, k).
+//   KC: k contiguous, element (r, k) at p[k + r*ld]
+//   RC: r contiguous, element (r, k) at p[r + k*ld]
+//   SYM: symmetric, lower triangle stored: (r, k) at p[r + k*ld] if r >= k, else p[k + r*ld]
+enum Layout { L_KC = 0, L_RC = 1, L_SYM = 2 };
+
+// Registers holding one operand tile between its global load and its LDS store.
+// Chunk c of the tile is E consecutive elements along the contiguous direction of
+// the layout it was loaded with (KC: along k; RC and SYM: along r).
+template <typename T, int R, int BK, int NT> struct TileRegs {
+    using vec = typename Mfma<T>::vec;
+    static constexpr int E = Mfma<T>::E;
+    static constexpr int CH = R * BK / E / NT;
+    static_assert(CH * E * NT == R * BK, "tile / thread count mismatch");
+    static constexpr int KCH = BK / E; // KC: chunks per row
+    static constexpr int RCH = R / E;  // RC: chunks per k column
+    vec v[CH];
+
+    // 16-byte loads of an interior tile (rows r0.., k0.. valid, source aligned)
+    __device__ __forceinline__ void load_vec(int lay, const T *p, i64 ld, i64 r0, i64 k0, int tid) {
+        if (lay == L_KC) {
 #pragma unroll
-        for (int i = 0; i < NA; ++i) {
-            const i64 m = m0 + tid / BK + (kThreads / BK) * i;
-            ra[i] = (kok && m < M) ? pa[k + m * lda] : T(0);
-        }
-    } else {
-        const int ml = tid % BM;
-        const i64 m = m0 + ml;
-        const bool mok = m < M;
-        const int kk0 = tid / BM;
-        const T *base = pa + m + (k0 + kk0) * lda;
-        const i64 step = (i64)(kThreads / BM) * lda;
+            for (int q = 0; q < CH; ++q) {
+                const int c = tid + q * NT;
+                const int row = c / KCH, kc = c % KCH;
+                v[q] = *reinterpret_cast<const vec *>(p + (r0 + row) * ld + k0 + kc * E);
+            }
+        } else {
 #pragma unroll
-        for (int i = 0; i < NA; ++i) {
-            const i64 k = k0 + kk0 + (kThreads / BM) * i;
-            if (pat == 0) {
-                ra[i] = (mok && k < kend) ? base[i * step] : T(0);
-            } else {
-                const i64 idx = (m >= k) ? m + k * lda : k + m * lda;
-                ra[i] = (mok && k < kend) ? pa[idx] : T(0);
+            for (int q = 0; q < CH; ++q) {
+                const int c = tid + q * NT;
+                const int kk = c / RCH, rc = c % RCH;
+                v[q] = *reinterpret_cast<const vec *>(p + (k0 + kk) * ld + r0 + rc * E);
             }
         }
     }
-}
-
-template <typename T, int BM, int NA>
-__device__ __forceinline__ void store_a(T (*sA)[BM + 1], const T (&ra)[NA], int pat, int tid) {
-    if (pat == 1) {
+    // per-element loads with bounds (rows < rlim, k < klim read, the rest zero)
+    __device__ __forceinline__ void load_scalar(int lay, const T *p, i64 ld, i64 r0, i64 k0, i64 rlim,
+                                                i64 klim, int tid) {
 #pragma unroll
-        for (int i = 0; i < NA; ++i) sA[tid % BK][tid / BK + (kThreads / BK) * i] = ra[i];
-    } else {
+        for (int q = 0; q < CH; ++q) {
+            const int c = tid + q * NT;
 #pragma unroll
-        for (int i = 0; i < NA; ++i) sA[tid / BM + (kThreads / BM) * i][tid % BM] = ra[i];
-    }
-}
-
-// B tile (BK x BN at columns n0): pattern 0 k-contiguous (p[k + n*ld]), 1 n-contiguous.
-template <typename T, int BN, int NB>
-__device__ __forceinline__ void load_b(T (&rb)[NB], int pat, const T *pb, i64 ldb, i64 n0, i64 k0,
-                                       i64 kend, i64 N, int tid) {
-    if (pat == 0) {
-        const int kk = tid % BK;
-        const i64 k = k0 + kk;
-        const bool kok = k < kend;
-        const T *base = pb + k + (n0 + tid / BK) * ldb;
-        const i64 step = (i64)(kThreads / BK) * ldb;
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            const i64 n = n0 + tid / BK + (kThreads / BK) * i;
-            rb[i] = (kok && n < N) ? base[i * step] : T(0);
-        }
-    } else {
-        const int nl = tid % BN;
-        const i64 n = n0 + nl;
-        const bool nok = n < N;
-        const int kk0 = tid / BN;
-        const T *base = pb + n + (k0 + kk0) * ldb;
-        const i64 step = (i64)(kThreads / BN) * ldb;
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            const i64 k = k0 + kk0 + (kThreads / BN) * i;
-            rb[i] = (nok && k < kend) ? base[i * step] : T(0);
+            for (int e = 0; e < E; ++e) {
+                i64 r, k;
+                if (lay == L_KC) {
+                    r = r0 + c / KCH;
+                    k = k0 + (c % KCH) * E + e;
+                } else {
+                    r = r0 + (c % RCH) * E + e;
+                    k = k0 + c / RCH;
+                }
+                T x = T(0);
+                if (r < rlim && k < klim) {
+                    const i64 idx = (lay == L_KC) ? k + r * ld
+                                    : (lay == L_RC || r >= k) ? r + k * ld
+                                                              : k + r * ld;
+                    x = p[idx];
+                }
+                v[q][e] = x;
+            }
         }
     }
-}
-
-template <typename T, int BN, int NB>
-__device__ __forceinline__ void store_b(T (*sB)[BN + 1], const T (&rb)[NB], int pat, int tid) {
-    if (pat == 0) {
+    // into LDS s[row][k] (row stride RS elements)
+    template <int RS> __device__ __forceinline__ void store(int lay, T *s, int tid) const {
+        if (lay == L_KC) {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) sB[tid % BK][tid / BK + (kThreads / BK) * i] = rb[i];
-    } else {
+            for (int q = 0; q < CH; ++q) {
+                const int c = tid + q * NT;
+                *reinterpret_cast<vec *>(s + (c / KCH) * RS + (c % KCH) * E) = v[q];
+            }
+        } else {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) sB[tid / BN + (kThreads / BN) * i][tid % BN] = rb[i];
+            for (int q = 0; q < CH; ++q) {
+                const int c = tid + q * NT;
+                const int kk = c / RCH, r = (c % RCH) * E;
+#pragma unroll
+                for (int e = 0; e < E; ++e) s[(r + e) * RS + kk] = v[q][e];
+            }
+        }
     }
-}
+};
 
-// Block tile BM x BN, K step BK, 4 waves (2 x 2) each owning (BM/2) x (BN/2).
-// __launch_bounds__(256, 2): at most 256 VGPRs, so two waves share each SIMD and
-// one hides the other's LDS and barrier latency.
-template <typename T, int BM, int BN, int AMODE, int BMODE>
-__global__ __launch_bounds__(kThreads, 2) void gemm_tile_kernel(const KParams<T> p) {
-    constexpr int TM = BM / 2, TN = BN / 2;
+// C tile BM x BN of C = alpha op(A) op(B) + beta C over the split's K range.
+template <typename T, int BM, int BN, int WM, int WN, int AMODE, int BMODE, int MINB>
+__global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_tile_kernel(const KParams<T> p) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int E = Mfma<T>::E;
+    constexpr int BK = Mfma<T>::BK;
+    constexpr int RS = BK + E; // LDS row stride in elements: 16 bytes of padding
+    constexpr int TM = BM / WM, TN = BN / WN;
     constexpr int FM = TM / 16, FN = TN / 16;
-    constexpr int NA = BM * BK / kThreads; // A elements per thread per k-tile
-    constexpr int NB = BN * BK / kThreads;
+    constexpr int KG = 4 * E; // k covered by one fragment read
     using V4 = typename Mfma<T>::v4;
+    using vec = typename Mfma<T>::vec;
 
-    __shared__ T sA[2][BK][BM + 1];
-    __shared__ T sB[2][BK][BN + 1];
+    __shared__ __attribute__((aligned(16))) T sA[2][BM * RS];
+    __shared__ __attribute__((aligned(16))) T sB[2][BN * RS];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WM, wn = wave / WM;
 
     // XCD-contiguous remap of the block index
     const int nbk = gridDim.x, bid = blockIdx.x;
-    const int xcd = bid & 7, q = nbk >> 3, rmd = nbk & 7;
-    const int t = xcd * q + min(xcd, rmd) + (bid >> 3);
+    const int xcd = bid & 7, q8 = nbk >> 3, rmd = nbk & 7;
+    const int t = xcd * q8 + min(xcd, rmd) + (bid >> 3);
     int tm, tn;
     tile_coords(t, p.tiles_m, p.tiles_n, p.lower, tm, tn);
     const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
@@ -222,71 +236,95 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_tile_kernel(const KParams<T>
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
 
-    T ra[NA], rb[NB];
-    int patA = 0, patB = 0;
+    TileRegs<T, BM, BK, NT> ra;
+    TileRegs<T, BN, BK, NT> rb;
+    const bool m_in = m0 + BM <= p.M, n_in = n0 + BN <= p.N;
+    int layA = L_RC, layB = L_KC;
     auto load_tiles = [&](i64 k0) {
-        // A operand source and pattern for this k-tile
-        const T *pa = p.a0;
-        i64 lda = p.lda0, ko = 0;
-        if constexpr (AMODE == A_N) {
-            patA = 0;
-        } else if constexpr (AMODE == A_T) {
-            patA = 1;
-        } else if constexpr (AMODE == A_SYM) {
-            patA = (m0 >= k0 + BK - 1) ? 0 : (m0 + BM - 1 <= k0) ? 1 : 2;
-        } else {
-            patA = 0;
-            if (k0 >= p.asplit) {
-                pa = p.a1;
-                lda = p.lda1;
-                ko = p.asplit;
+        const bool k_in = k0 + BK <= kend;
+        // ---- A
+        {
+            const T *pa = p.a0;
+            i64 lda = p.lda0, ko = 0;
+            bool fast = m_in && k_in && p.avec;
+            if constexpr (AMODE == A_N) {
+                layA = L_RC;
+            } else if constexpr (AMODE == A_T) {
+                layA = L_KC;
+            } else if constexpr (AMODE == A_SYM) {
+                if (m0 >= k0 + BK - 1) {
+                    layA = L_RC; // below the diagonal
+                } else if (m0 + BM - 1 <= k0) {
+                    layA = L_KC; // above: the transpose of stored lower entries
+                } else {
+                    layA = L_SYM;
+                    fast = false;
+                }
+            } else {
+                layA = L_RC;
+                if (k0 >= p.asplit) {
+                    pa = p.a1;
+                    lda = p.lda1;
+                    ko = p.asplit;
+                }
             }
+            if (fast) ra.load_vec(layA, pa, lda, m0, k0 - ko, tid);
+            else ra.load_scalar(layA, pa, lda, m0, k0 - ko, p.M, kend - ko, tid);
+            if (layA == L_SYM) layA = L_RC; // stored with the RC chunk mapping
         }
-        load_a<T, BM, NA>(ra, patA, pa, lda, m0, k0 - ko, kend - ko, p.M, tid);
-        const T *pb = p.b0;
-        i64 ldb = p.ldb0, kob = 0;
-        if constexpr (BMODE == B_N) {
-            patB = 0;
-        } else if constexpr (BMODE == B_T) {
-            patB = 1;
-        } else {
-            patB = 1;
-            if (k0 >= p.bsplit) {
-                pb = p.b1;
-                ldb = p.ldb1;
-                kob = p.bsplit;
+        // ---- B (rows of the tile = n)
+        {
+            const T *pb = p.b0;
+            i64 ldb = p.ldb0, ko = 0;
+            const bool fast = n_in && k_in && p.bvec;
+            if constexpr (BMODE == B_N) {
+                layB = L_KC;
+            } else if constexpr (BMODE == B_T) {
+                layB = L_RC;
+            } else {
+                layB = L_RC;
+                if (k0 >= p.bsplit) {
+                    pb = p.b1;
+                    ldb = p.ldb1;
+                    ko = p.bsplit;
+                }
             }
+            if (fast) rb.load_vec(layB, pb, ldb, n0, k0 - ko, tid);
+            else rb.load_scalar(layB, pb, ldb, n0, k0 - ko, p.N, kend - ko, tid);
         }
-        load_b<T, BN, NB>(rb, patB, pb, ldb, n0, k0 - kob, kend - kob, p.N, tid);
     };
 
     if (nkt > 0) {
         load_tiles(kbeg);
-        store_a<T, BM, NA>(sA[0], ra, patA, tid);
-        store_b<T, BN, NB>(sB[0], rb, patB, tid);
+        ra.template store<RS>(layA, sA[0], tid);
+        rb.template store<RS>(layB, sB[0], tid);
     }
     __syncthreads();
 
+    const int li = lane & 15, lj = lane >> 4;
     for (int kt = 0; kt < nkt; ++kt) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nkt;
         if (more) load_tiles(kbeg + (i64)(kt + 1) * BK);
+        const T *a_s = sA[cur] + (wm * TM + li) * RS + lj * E;
+        const T *b_s = sB[cur] + (wn * TN + li) * RS + lj * E;
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
-            const int kr = ks * 4 + (lane >> 4);
-            T af[FM], bf[FN];
+        for (int kg = 0; kg < BK / KG; ++kg) {
+            vec af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = sA[cur][kr][wm * TM + i * 16 + (lane & 15)];
+            for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const vec *>(a_s + i * 16 * RS + kg * KG);
 #pragma unroll
-            for (int j = 0; j < FN; ++j) bf[j] = sB[cur][kr][wn * TN + j * 16 + (lane & 15)];
+            for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const vec *>(b_s + j * 16 * RS + kg * KG);
 #pragma unroll
-            for (int i = 0; i < FM; ++i)
+            for (int e = 0; e < E; ++e)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(bf[j], af[i], acc[i][j]);
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(bf[j][e], af[i][e], acc[i][j]);
         }
         if (more) {
-            store_a<T, BM, NA>(sA[cur ^ 1], ra, patA, tid);
-            store_b<T, BN, NB>(sB[cur ^ 1], rb, patB, tid);
+            ra.template store<RS>(layA, sA[cur ^ 1], tid);
+            rb.template store<RS>(layB, sB[cur ^ 1], tid);
         }
         __syncthreads();
     }
@@ -296,7 +334,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_tile_kernel(const KParams<T>
     T *part = partial ? p.part + (i64)blockIdx.y * p.M * p.N : nullptr;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-        const i64 m = m0 + wm * TM + i * 16 + (lane & 15);
+        const i64 m = m0 + wm * TM + i * 16 + li;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
@@ -331,29 +369,46 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int s
     *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
 }
 
-template <typename T, int BM, int BN, int AM, int BMD>
-int launch_modes(hipStream_t s, KParams<T> &p, int splits, int ntiles) {
+// Tile configurations: (BM, BN, WM, WN, blocks per CU the launch bounds ask for).
+enum Cfg { C128x128 = 0, C128x64 = 1, C64x128 = 2, C64x64 = 3, C256x128 = 4 };
+
+template <typename T, int BM, int BN, int WM, int WN, int MINB, int AM, int BMD>
+int launch_one(hipStream_t s, const KParams<T> &p, int splits, int ntiles) {
     dim3 grid((unsigned)ntiles, (unsigned)splits);
-    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, AM, BMD>), grid, dim3(kThreads), 0, s, p);
+    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, WM, WN, AM, BMD, MINB>), grid, dim3(64 * WM * WN), 0, s, p);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
 }
 
-template <typename T, int BM, int BN>
-int launch_cfg(hipStream_t s, KParams<T> &p, int splits, int ntiles, int am, int bm) {
-#define CUEV_GEMM_B(AMV)                                                                           \
-    switch (bm) {                                                                                  \
-    case B_N: return launch_modes<T, BM, BN, AMV, B_N>(s, p, splits, ntiles);                      \
-    case B_T: return launch_modes<T, BM, BN, AMV, B_T>(s, p, splits, ntiles);                      \
-    default: return launch_modes<T, BM, BN, AMV, B_CATT>(s, p, splits, ntiles);                    \
+template <typename T, int AM, int BMD>
+int launch_cfg(hipStream_t s, const KParams<T> &p, int splits, int ntiles, int cfg) {
+    switch (cfg) {
+    case C128x128: return launch_one<T, 128, 128, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
+    case C128x64: return launch_one<T, 128, 64, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
+    case C64x128: return launch_one<T, 64, 128, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
+    case C256x128: return launch_one<T, 256, 128, 4, 2, 1, AM, BMD>(s, p, splits, ntiles);
+    default: return launch_one<T, 64, 64, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
     }
-    switch (am) {
-    case A_N: CUEV_GEMM_B(A_N)
-    case A_T: CUEV_GEMM_B(A_T)
-    case A_SYM: CUEV_GEMM_B(A_SYM)
-    default: CUEV_GEMM_B(A_CAT)
-    }
-#undef CUEV_GEMM_B
+}
+
+template <typename T>
+int launch_modes(hipStream_t s, const KParams<T> &p, int splits, int ntiles, int cfg, int am, int bm) {
+    if (am == A_N && bm == B_N) return launch_cfg<T, A_N, B_N>(s, p, splits, ntiles, cfg);
+    if (am == A_N && bm == B_T) return launch_cfg<T, A_N, B_T>(s, p, splits, ntiles, cfg);
+    if (am == A_T && bm == B_N) return launch_cfg<T, A_T, B_N>(s, p, splits, ntiles, cfg);
+    if (am == A_T && bm == B_T) return launch_cfg<T, A_T, B_T>(s, p, splits, ntiles, cfg);
+    if (am == A_SYM && bm == B_N) return launch_cfg<T, A_SYM, B_N>(s, p, splits, ntiles, cfg);
+    if (am == A_CAT && bm == B_N) return launch_cfg<T, A_CAT, B_N>(s, p, splits, ntiles, cfg);
+    if (am == A_CAT && bm == B_CATT) return launch_cfg<T, A_CAT, B_CATT>(s, p, splits, ntiles, cfg);
+    CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "gemm: operand mode combination (%d, %d) not implemented", am, bm);
+}
+
+int cfg_bm(int cfg) { return (cfg == C128x128 || cfg == C128x64) ? 128 : (cfg == C256x128) ? 256 : 64; }
+int cfg_bn(int cfg) { return (cfg == C128x128 || cfg == C64x128 || cfg == C256x128) ? 128 : 64; }
+
+template <typename T> bool vec_ok(const T *p, i64 ld) {
+    constexpr int E = 16 / (int)sizeof(T);
+    return p == nullptr || ((reinterpret_cast<uintptr_t>(p) & 15) == 0 && ld % E == 0);
 }
 
 } // namespace
@@ -366,75 +421,10 @@ size_t gemm_scratch_bytes(i64 M, i64 N, size_t elem) {
     return (size_t)3 * 512 * 128 * 128 * elem;
 }
 
-namespace {
-
-// rocBLAS handles, one per stream (a handle's workspace must not be shared by
-// work queued on two streams).
-std::mutex g_vendor_mu;
-std::vector<std::pair<hipStream_t, rocblas_handle>> g_vendor;
-
-rocblas_handle vendor_handle(hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_vendor_mu);
-    for (auto &e : g_vendor)
-        if (e.first == s) return e.second;
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    if (rocblas_set_stream(h, s) != rocblas_status_success) {
-        rocblas_destroy_handle(h);
-        return nullptr;
-    }
-    g_vendor.emplace_back(s, h);
-    return h;
-}
-
-inline rocblas_status vendor_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n,
-                                  int k, const double *alpha, const double *A, int lda, const double *B, int ldb,
-                                  const double *beta, double *C, int ldc) {
-    return rocblas_dgemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-}
-inline rocblas_status vendor_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n,
-                                  int k, const float *alpha, const float *A, int lda, const float *B, int ldb,
-                                  const float *beta, float *C, int ldc) {
-    return rocblas_sgemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-}
-
-} // namespace
-
-int gemm_vendor_attach(hipStream_t s) {
-    if (!vendor_handle(s)) CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "gemm: rocblas handle creation failed");
-    return CUEV_STATUS_SUCCESS;
-}
-
-void gemm_vendor_release(hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_vendor_mu);
-    for (size_t i = 0; i < g_vendor.size(); ++i)
-        if (g_vendor[i].first == s) {
-            rocblas_destroy_handle(g_vendor[i].second);
-            g_vendor.erase(g_vendor.begin() + (long)i);
-            return;
-        }
-}
-
-namespace {
-
-template <typename T>
-int gemm_vendor(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A, const Operand<T> &B,
-                T beta, T *C, i64 ldc) {
-    rocblas_handle h = vendor_handle(s);
-    if (!h) CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "gemm: rocblas handle creation failed");
-    const rocblas_operation ta = A.trans ? rocblas_operation_transpose : rocblas_operation_none;
-    const rocblas_operation tb = B.trans ? rocblas_operation_transpose : rocblas_operation_none;
-    const rocblas_status st = vendor_gemm(h, ta, tb, (int)M, (int)N, (int)K, &alpha, A.p0, (int)A.ld0, B.p0,
-                                          (int)B.ld0, &beta, C, (int)ldc);
-    if (st != rocblas_status_success) CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "gemm: rocblas_?gemm returned status %d", (int)st);
-    return CUEV_STATUS_SUCCESS;
-}
-
-} // namespace
-
 template <typename T>
 int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
             const Operand<T> &B, T beta, T *C, i64 ldc, int lower, Scratch scratch) {
+    constexpr int BK = Mfma<T>::BK;
     if (M < 0 || N < 0 || K < 0)
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: negative size M=%lld N=%lld K=%lld",
                   (long long)M, (long long)N, (long long)K);
@@ -443,10 +433,6 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     if ((A.split >= 0 && A.split % BK) || (B.split >= 0 && B.split % BK))
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: K split must be a multiple of %d", BK);
     if (ldc < M) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: ldc < M");
-    const i64 imax = 0x7fffffffLL;
-    if (scratch.vendor && !lower && A.split < 0 && B.split < 0 && !A.sym && std::min(std::min(M, N), K) >= 512 &&
-        (double)M * N * K >= 8.6e9 && M <= imax && N <= imax && K <= imax && A.ld0 <= imax && B.ld0 <= imax && ldc <= imax)
-        return gemm_vendor<T>(s, M, N, K, alpha, A, B, beta, C, ldc);
 
     KParams<T> p{};
     p.a0 = A.p0;
@@ -454,14 +440,11 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     p.lda0 = A.ld0;
     p.lda1 = A.ld1;
     p.asplit = A.split;
-    p.atrans = A.trans;
-    p.asym = A.sym;
     p.b0 = B.p0;
     p.b1 = B.p1;
     p.ldb0 = B.ld0;
     p.ldb1 = B.ld1;
     p.bsplit = B.split;
-    p.btrans = B.trans;
     p.C = C;
     p.ldc = ldc;
     p.M = M;
@@ -470,27 +453,34 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     p.alpha = alpha;
     p.beta = beta;
     p.lower = lower;
+    p.avec = vec_ok(A.p0, A.ld0) && vec_ok(A.p1, A.ld1);
+    p.bvec = vec_ok(B.p0, B.ld0) && vec_ok(B.p1, B.ld1);
 
     // tile shape
-    int BMs = (M > 64) ? 128 : 64;
-    int BNs = (N > 64) ? 128 : 64;
-    if (lower) BMs = BNs = (N >= 1024) ? 128 : 64;
-    if (scratch.small_tiles) BMs = BNs = 64;
-    // symmetric A times a panel-wide B (the band reduction's A22 W): bandwidth-bound on
-    // A, so small tiles and deep split-K keep more loads in flight (measured at
-    // b = 64: 0.87 -> 0.70 ms for 16320 rows, 0.10 -> 0.045 ms for 3000)
-    const bool sym_panel = A.sym && N <= 64;
-    if (sym_panel) BMs = 64;
     const int cus = device_cu_count();
     auto count_tiles = [&](int bm, int bn) -> i64 {
         const i64 tm = ceil_div(M, bm), tn = ceil_div(N, bn);
         return lower ? tm * (tm + 1) / 2 : tm * tn;
     };
-    if (!lower && BMs == 128 && BNs == 128 && count_tiles(128, 128) < cus) {
-        // few large tiles: trade tile size for parallelism
-        if (count_tiles(64, 128) >= cus || M < N) BMs = 64;
-        else BNs = 64;
+    int cfg;
+    const bool sym_panel = A.sym && N <= 64;
+    if (scratch.small_tiles) {
+        cfg = C64x64;
+    } else if (lower) {
+        cfg = (N >= 1024) ? C128x128 : C64x64;
+    } else if (sym_panel) {
+        // symmetric A times a panel-wide B (the band reduction's A22 W): bandwidth-bound
+        // on A, so small tiles and deep split-K keep more loads in flight
+        cfg = C64x64;
+    } else {
+        const int bm = (M > 64) ? 128 : 64, bn = (N > 64) ? 128 : 64;
+        cfg = (bm == 128 && bn == 128) ? C128x128 : (bm == 128) ? C128x64 : (bn == 128) ? C64x128 : C64x64;
+        if (cfg == C128x128 && count_tiles(128, 128) < cus) {
+            // few large tiles: trade tile size for parallelism
+            cfg = (count_tiles(64, 128) >= cus || M < N) ? C64x128 : C128x64;
+        }
     }
+    const int BMs = cfg_bm(cfg), BNs = cfg_bn(cfg);
     const i64 ntiles = count_tiles(BMs, BNs);
     if (ntiles > 0x7fffffffLL) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "gemm: too many tiles");
     p.tiles_m = (int)ceil_div(M, BMs);
@@ -528,12 +518,7 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
         if ((A.sym && (A.split >= 0 || A.trans)) || (A.split >= 0 && A.trans) ||
             (B.split >= 0 && !B.trans) || B.sym)
             CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "gemm: operand mode combination not implemented");
-        int st;
-        if (BMs == 128 && BNs == 128) st = launch_cfg<T, 128, 128>(s, p, splits, (int)ntiles, am, bm);
-        else if (BMs == 128) st = launch_cfg<T, 128, 64>(s, p, splits, (int)ntiles, am, bm);
-        else if (BNs == 128) st = launch_cfg<T, 64, 128>(s, p, splits, (int)ntiles, am, bm);
-        else st = launch_cfg<T, 64, 64>(s, p, splits, (int)ntiles, am, bm);
-        if (st) return st;
+        CUEV_TRY(launch_modes<T>(s, p, splits, (int)ntiles, cfg, am, bm));
     }
     if (p.part != nullptr || splits == 0) {
         const i64 total = M * N;
@@ -549,7 +534,7 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
 template <typename T>
 int syr2k_lower(hipStream_t s, i64 n, i64 k, T alpha, const T *X, i64 ldx, const T *Y, i64 ldy,
                 T beta, T *C, i64 ldc, Scratch scratch) {
-    if (k % BK == 0) {
+    if (k % Mfma<T>::BK == 0) {
         // [X | Y] * [Y | X]^T as one product with K = 2k
         Operand<T> a = opnd_cat(X, ldx, Y, ldy, k, 0);
         Operand<T> b = opnd_cat(Y, ldy, X, ldx, k, 1);

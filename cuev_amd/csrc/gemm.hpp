@@ -47,24 +47,13 @@ inline Operand<T> opnd_cat(const T *p0, i64 ld0, const T *p1, i64 ld1, i64 split
 }
 
 // Device scratch the engine may use for split-K partial sums. small_tiles: use
-// only 64 x 64 block tiles (about 100 VGPRs instead of about 220), for products
-// meant to run beside a kernel that holds most of each SIMD's registers. vendor:
-// a large plain product (no K-concatenation, no symmetric operand, all of C, every
-// dimension >= 512, M N K >= 8.6e9) goes to rocBLAS, which runs such shapes at
-// 67-76 TF/s FP64 on gfx950 against 50-62 for this engine (n^3 products: 75 vs 62;
-// rank-512 updates of an n x n matrix: 67-69 vs 50-53).
+// only 64 x 64 block tiles (fewer registers per wave), for products meant to run
+// beside a kernel that holds most of each SIMD's registers.
 struct Scratch {
     void *ptr = nullptr;
     size_t bytes = 0;
     int small_tiles = 0;
-    int vendor = 0;
 };
-
-// rocBLAS handle bound to stream s, created on first use; released (all of them)
-// by gemm_vendor_release_all at library teardown, or for one stream by
-// gemm_vendor_release before the stream is destroyed.
-int gemm_vendor_attach(hipStream_t s);
-void gemm_vendor_release(hipStream_t s);
 
 // C <- alpha * op(A) op(B) + beta * C  (lower != 0: only entries row >= col of C
 // are written; requires M == N). beta == 0 never reads C.

@@ -518,8 +518,6 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     // FP64 products share the FP32 workspace's split-K scratch (the engine limits
     // the split count to what fits)
     Scratch gs64{ws.gscratch, ws.gscratch_bytes};
-    Scratch gs64v = gs64; // plain products with the GPU to themselves: rocBLAS when large
-    gs64v.vendor = 1;
     const int max_sweeps = opt.refine_sweeps > 0 ? opt.refine_sweeps : 3;
     const double tol = opt.refine_tol > 0 ? opt.refine_tol : 2e-14;
     std::vector<double> hp(3 * (size_t)kRedBlocks);
@@ -607,7 +605,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         CUEV_LAUNCH_CHECK();
         // next iterate X + X E into `other` (A X is no longer needed)
         CUEV_TRY(copy2d<double>(s, X, n, other, n, n, n));
-        CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, X, n, mb.G, n, 1.0, other, n, gs64v));
+        CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, X, n, mb.G, n, 1.0, other, n, gs64));
         CUEV_HIP(hipMemcpyAsync(&nclus, mb.count, 4, hipMemcpyDeviceToHost, s));
         X = other;
         ++sweeps;

@@ -24,6 +24,7 @@
 #include "gemm.hpp"
 #include "lapack.hpp"
 #include "profile.hpp"
+#include "tridi_values.hpp"
 #include "util.hpp"
 
 namespace cuev {
@@ -187,11 +188,8 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
             CUEV_HIP(hipMemcpyAsync(ws.W, ctx.h_z, sizeof(T) * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipEventRecord(ev[7], s));
         {
-            // a plain product with the GPU to itself: the vendor library's tiles
             prof::Scope sc(s, "bt.gemm", 2.0 * (double)n * n * n);
-            Scratch gv = gs;
-            gv.vendor = 1;
-            CUEV_TRY(gemm<T>(s, 1, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gv));
+            CUEV_TRY(gemm<T>(s, 1, 0, n, n, n, T(1), ws.M, ws.ld, Qd, n, T(0), evec, n, gs));
         }
         CUEV_HIP(hipEventRecord(ev[8], s));
     } else {
@@ -229,6 +227,46 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
     return CUEV_STATUS_SUCCESS;
 }
 
+template <typename T>
+int solve_values(Context<T> &ctx, T *A, int il, int iu, T *eval, cuev_solve_timer *timer) {
+    Workspace<T> &ws = ctx.ws;
+    const int n = ws.n;
+    hipStream_t s = ws.stream;
+    hipEvent_t *ev = ctx.ev;
+    const auto wall0 = clk::now();
+    CUEV_HIP(hipEventRecord(ev[0], s));
+    CUEV_TRY(dbbr_reduce<T>(ws, A, ws.B));
+    CUEV_HIP(hipEventRecord(ev[1], s));
+    CUEV_TRY(bc_chase<T>(ws, ws.B, ws.d, ws.e));
+    CUEV_HIP(hipEventRecord(ev[2], s));
+    {
+        prof::Scope sc(s, "tv.multisect");
+        CUEV_TRY(tridi_values<T>(s, ws.d, ws.e, n, il, iu, eval));
+    }
+    CUEV_HIP(hipEventRecord(ev[3], s));
+    // (d, e) to the host for the finiteness check (the count of a NaN pivot is
+    // meaningless, so a non-finite input must be reported, not returned)
+    CUEV_HIP(hipMemcpyAsync(ctx.h_d, ws.d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipMemcpyAsync(ctx.h_e, ws.e, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    CUEV_HIP(hipStreamSynchronize(s));
+    CUEV_TRY(workspace_check_error(ws));
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite((double)ctx.h_d[i]) || (i < n - 1 && !std::isfinite((double)ctx.h_e[i])))
+            CUEV_FAIL(CUEV_STATUS_NOT_FINITE,
+                      "tridiagonal matrix has a non-finite entry at %d (input not finite?)", i);
+    const auto wall1 = clk::now();
+    if (timer) {
+        std::memset(timer, 0, sizeof(*timer));
+        timer->dbbr_ms = ev_ms(ev[0], ev[1]);
+        timer->bc_ms = ev_ms(ev[1], ev[2]);
+        timer->dc_ms = ev_ms(ev[2], ev[3]);
+        timer->total_ms = (float)ms_since(wall0, wall1);
+    }
+    return CUEV_STATUS_SUCCESS;
+}
+
+template int solve_values<float>(Context<float> &, float *, int, int, float *, cuev_solve_timer *);
+template int solve_values<double>(Context<double> &, double *, int, int, double *, cuev_solve_timer *);
 template int tridi_dc<float>(Context<float> &, float *, float *, float *, float *, i64, double *);
 template int tridi_dc<double>(Context<double> &, double *, double *, double *, double *, i64,
                               double *);

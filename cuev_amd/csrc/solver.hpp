@@ -15,6 +15,10 @@ int tridi_dc(Context<T> &ctx, T *d, T *e, T *eval, T *evec, i64 ldv, double *dc_
 template <typename T>
 int back_transform(Context<T> &ctx, const T *Y, const T *W, const T *U, T *evec,
                    cuev_solve_timer *timer);
+// Eigenvalues il..iu only (0-based, ascending) into eval[0 .. iu-il]: DBBR, bulge
+// chasing, then Sturm multisection on the GPU; no back-transform.
+template <typename T>
+int solve_values(Context<T> &ctx, T *A, int il, int iu, T *eval, cuev_solve_timer *timer);
 // Full solve in the precision T.
 template <typename T>
 int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *timer, int bt_mode,

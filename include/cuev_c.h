@@ -149,6 +149,43 @@ int cuev_dsymm_eig_solve_ex(double *A, int n, double *eval, double *evec, hipStr
                             cuev_solve_timer *timer, const cuev_options *opt);
 
 /* ------------------------------------------------------------------------ */
+/* Eigenvalues only (no reference counterpart: the reference computes every  */
+/* eigenpair, src/cuda/tridi.cu:12-13). DBBR -> bulge chasing -> Sturm-       */
+/* sequence multisection on the GPU; no back-transform, no D&C. A is read     */
+/* (lower triangle) and overwritten as by the full solve. Blocking.           */
+/*   _values:    all n eigenvalues, ascending, into eval[0..n-1].             */
+/*   _values_ex: eigenvalues il..iu (0-based, inclusive, ascending) into      */
+/*               eval[0..iu-il]; opt may be NULL (bandwidth and block used;   */
+/*               precision must be native).                                   */
+/* ------------------------------------------------------------------------ */
+int cuev_ssymm_eig_values(float *A, int n, float *eval, hipStream_t stream, cuev_solve_timer *timer);
+int cuev_dsymm_eig_values(double *A, int n, double *eval, hipStream_t stream, cuev_solve_timer *timer);
+int cuev_ssymm_eig_values_ex(float *A, int n, int il, int iu, float *eval, hipStream_t stream,
+                             cuev_solve_timer *timer, const cuev_options *opt);
+int cuev_dsymm_eig_values_ex(double *A, int n, int il, int iu, double *eval, hipStream_t stream,
+                             cuev_solve_timer *timer, const cuev_options *opt);
+/* Eigenvalues il..iu of the symmetric tridiagonal matrix with diagonal d (n) and
+ * off-diagonal e (n-1 used), all device pointers; w[0..iu-il] ascending.
+ * Asynchronous on stream; uses no scratch memory. */
+int cuev_stridi_values(const float *d, const float *e, int n, int il, int iu, float *w, hipStream_t stream);
+int cuev_dtridi_values(const double *d, const double *e, int n, int il, int iu, double *w,
+                       hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched small eigenproblems (no reference counterpart: the reference      */
+/* solves one matrix per call, include/cuev.h:31-33). batch matrices of order */
+/* n <= 256, stored one after another (each n x n column-major, ld n; lower   */
+/* triangle read; A is overwritten); eval receives batch x n eigenvalues      */
+/* (ascending per matrix), evec batch eigenvector matrices (columns). One      */
+/* workgroup per matrix, cyclic Jacobi. Blocking. Returns NOT_CONVERGED if a   */
+/* matrix did not converge (count in *n_not_converged when not NULL).          */
+/* ------------------------------------------------------------------------ */
+int cuev_ssymm_eig_batched(float *A, int n, int batch, float *eval, float *evec, hipStream_t stream,
+                           int *n_not_converged);
+int cuev_dsymm_eig_batched(double *A, int n, int batch, double *eval, double *evec, hipStream_t stream,
+                           int *n_not_converged);
+
+/* ------------------------------------------------------------------------ */
 /* Backward pass of the eigendecomposition (no reference counterpart).        */
 /*   Abar = V (diag(lambar) + F o skew) V^T,  skew = (V^T Vbar - Vbar^T V)/2,  */
 /*   F_ij = 1/(lambda_j - lambda_i) for i != j, 0 on the diagonal and where     */
