@@ -95,6 +95,7 @@ def _declare(L):
         getattr(L, f"cuev_{pre}tridi_values").argtypes = [P, P, I, I, I, P, P]
         getattr(L, f"cuev_{pre}symm_eig_batched").argtypes = [P, I, I, P, P, P, ctypes.POINTER(I)]
         getattr(L, f"cuev_{pre}symm_eig_backward").argtypes = [P, P, P, P, I, P, D, ctypes.POINTER(I), P]
+        getattr(L, f"cuev_{pre}symm_eig_backward_noalloc").argtypes = [P, P, P, P, I, P, D, ctypes.POINTER(I), P]
         getattr(L, f"cuev_{pre}_dbbr_pack").argtypes = [P, P, P, I, I]
         getattr(L, f"cuev_{pre}_dbbr_panel_qr").argtypes = [P, P, P, I, I]
         getattr(L, f"cuev_{pre}_dbbr_reduce").argtypes = [P, P, P]

@@ -17,7 +17,13 @@
 // Work buffers (C and T, n x n each, and the masked-pair counter):
 //  * cuev_{s,d}symm_eig_backward keeps them per device between calls (allocated
 //    on the first call of a size, grown when a larger n comes; a call that finds
-//    the cached buffer in use by another call allocates its own).
+//    the cached buffer in use by another call allocates its own);
+//  * cuev_{s,d}symm_eig_backward_noalloc allocates nothing: it uses the file-scope
+//    device array `scratch` below (n <= kNoAllocMaxN = 1024, 16 MiB per GPU for the
+//    life of the process), whose address is looked up once per device. Every call
+//    on one GPU shares that GPU's array, so calls that overlap on one GPU race; the
+//    counter is cleared (by a fill kernel) at the start of every call. With
+//    n_masked == NULL it only queues the work on the stream and returns.
 #include "backward.hpp"
 
 #include <mutex>
@@ -93,6 +99,32 @@ BwdBuf g_bwd_idle[64];
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// ---- the allocation-free entry point's device array
+constexpr int kNoAllocMaxN = 1024;
+constexpr size_t kNoAllocBytes = 2 * (size_t)kNoAllocMaxN * kNoAllocMaxN * sizeof(double) + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scr_mu;
+void *g_scr_ptr[64] = {};
+
+// Device address of `scratch` on the current device, looked up on the first call there.
+int scratch_ptr(void **out) {
+    int dev = 0;
+    CUEV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "eig_backward_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (!g_scr_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            CUEV_FAIL(CUEV_STATUS_HIP_ERROR, "eig_backward_noalloc: hipGetSymbolAddress(scratch) -> %s",
+                      hipGetErrorString(e));
+        g_scr_ptr[dev] = p;
+    }
+    *out = g_scr_ptr[dev];
+    return CUEV_STATUS_SUCCESS;
+}
+
 } // namespace
 
 template <typename T>
@@ -162,9 +194,33 @@ void backward_release_all() {
     (void)hipSetDevice(cur);
 }
 
+template <typename T>
+int eig_backward_noalloc(const T *V, const T *lam, const T *Vbar, const T *lambar, int n, T *Abar,
+                         double gap_tol, int *n_masked_host, hipStream_t s) {
+    CUEV_TRY(check_args(V, lam, Abar, n));
+    if (n > kNoAllocMaxN)
+        CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT,
+                  "eig_backward_noalloc: n=%d exceeds the fixed work array's limit n <= %d", n, kNoAllocMaxN);
+    void *p = nullptr;
+    CUEV_TRY(scratch_ptr(&p));
+    auto *base = static_cast<unsigned char *>(p);
+    const size_t mat = (size_t)kNoAllocMaxN * kNoAllocMaxN * sizeof(double);
+    T *C = reinterpret_cast<T *>(base);
+    T *Tm = reinterpret_cast<T *>(base + mat);
+    int *dmask = reinterpret_cast<int *>(base + 2 * mat);
+    // no split-K partials: the products run as single passes over K. Without
+    // n_masked the call returns once the work is queued on s.
+    return backward_core<T>(V, lam, Vbar, lambar, n, Abar, gap_tol, n_masked_host, s, C, Tm, dmask, Scratch{},
+                            false);
+}
+
 template int eig_backward<float>(const float *, const float *, const float *, const float *, int,
                                  float *, double, int *, hipStream_t);
 template int eig_backward<double>(const double *, const double *, const double *, const double *,
                                   int, double *, double, int *, hipStream_t);
+template int eig_backward_noalloc<float>(const float *, const float *, const float *, const float *, int,
+                                         float *, double, int *, hipStream_t);
+template int eig_backward_noalloc<double>(const double *, const double *, const double *, const double *,
+                                          int, double *, double, int *, hipStream_t);
 
 } // namespace cuev

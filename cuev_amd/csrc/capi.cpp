@@ -329,6 +329,19 @@ int cuev_dsymm_eig_backward(const double *V, const double *lambda, const double 
     return eig_backward<double>(V, lambda, Vbar, lambar, n, Abar, gap_tol, n_masked, stream);
 }
 
+int cuev_ssymm_eig_backward_noalloc(const float *V, const float *lambda, const float *Vbar,
+                                    const float *lambar, int n, float *Abar, double gap_tol,
+                                    int *n_masked, hipStream_t stream) {
+    detail::clear_error();
+    return eig_backward_noalloc<float>(V, lambda, Vbar, lambar, n, Abar, gap_tol, n_masked, stream);
+}
+int cuev_dsymm_eig_backward_noalloc(const double *V, const double *lambda, const double *Vbar,
+                                    const double *lambar, int n, double *Abar, double gap_tol,
+                                    int *n_masked, hipStream_t stream) {
+    detail::clear_error();
+    return eig_backward_noalloc<double>(V, lambda, Vbar, lambar, n, Abar, gap_tol, n_masked, stream);
+}
+
 int cuev_handle_create(cuev_handle_t *handle, int dtype, int n, int nbw, int nk, hipStream_t stream) {
     detail::clear_error();
     if (!handle) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "handle_create: null out pointer");

@@ -199,6 +199,20 @@ int cuev_ssymm_eig_backward(const float *V, const float *lambda, const float *Vb
 int cuev_dsymm_eig_backward(const double *V, const double *lambda, const double *Vbar,
                             const double *lambar, int n, double *Abar, double gap_tol,
                             int *n_masked, hipStream_t stream);
+/* Allocation-free variant of the two entry points above. No reference counterpart:
+ * it exists for callers that cannot allocate device memory (serving and training
+ * loops that forbid hipMalloc, harnesses that pre-plan every allocation). Same
+ * arguments and result, limited to n <= 1024 (INVALID_ARGUMENT above that). Its
+ * work buffers live in a fixed 16 MiB device array per GPU, held until the process
+ * ends and shared by every call on that GPU: calls that overlap on one GPU race.
+ * With n_masked == NULL the call returns once its work is queued on stream
+ * (asynchronous); otherwise it waits for the count, as the entry points above do. */
+int cuev_ssymm_eig_backward_noalloc(const float *V, const float *lambda, const float *Vbar,
+                                    const float *lambar, int n, float *Abar, double gap_tol,
+                                    int *n_masked, hipStream_t stream);
+int cuev_dsymm_eig_backward_noalloc(const double *V, const double *lambda, const double *Vbar,
+                                    const double *lambar, int n, double *Abar, double gap_tol,
+                                    int *n_masked, hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Stage-level ABI (mirrors cuev::kernels, include/cuda/kernels.cuh:28-157). */
