@@ -31,8 +31,9 @@
 //    sweep s-1, so a hop runs: reflector, write-back + flag, wait for sweep s-1,
 //    load the b new columns, rank-2 update. The reference waits for a margin of
 //    3b columns (bulge.cu:193-199); the critical path is ~ n times the lag.
-//  * Flags are agent-scope acquire/release atomics, every wait is bounded, and
-//    the grid never exceeds the number of co-resident workgroups, so the lowest
+//  * Hand-offs use agent-scope (write-through) band accesses and flags, with no L2
+//    write-back or invalidate (see ld_agent / st_agent below); every wait is bounded,
+//    and the grid never exceeds the number of co-resident workgroups, so the lowest
 //    unfinished sweep can always proceed.
 //  * U is stored shifted: U[(r - s - 1) + s*ldu] holds row r of sweep s's
 //    reflectors, so the back-transform reads aligned, contiguous segments.
@@ -48,6 +49,26 @@ namespace {
 
 constexpr int kBCThreads = 512; // 8 waves: two per SIMD, so LDS latency overlaps
 constexpr int kDone = 0x3fffffff; // prog value of a finished sweep
+
+// Band traffic between workgroups. Every global load and store of the packed band is
+// an agent-scope relaxed atomic access: on gfx950 these carry the sc1 bit, so a store
+// is written through to the device-coherent level (past the XCD-private L2) and a
+// load reads from it. A hand-off then needs no L2 write-back and no L2 invalidate:
+//   producer: band stores (sc1), s_waitcnt vmcnt(0) in every wave (the stores have
+//             completed at agent scope), workgroup barrier, one flag store (sc1);
+//   consumer: one thread polls the flag (sc1 loads), workgroup barrier, band loads
+//             (sc1), issued only after the flag was seen.
+// The reflectors (U) are read only by later kernels and use plain stores.
+template <typename T> __device__ __forceinline__ T ld_agent(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ void st_agent(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// progress flag store after wg_publish_fence (every wave's band stores complete)
+__device__ __forceinline__ void flag_store_agent(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 template <typename T, int B>
 __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 ldu, int n,
@@ -74,6 +95,11 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // window element A(i, j) (i >= j, both in the current window)
     auto W_ = [&](int i, int j) -> T & { return win[(j & (NSLOT - 1)) * WLD + (i - j)]; };
+    // symmetric read A(i, k) = A(max, min): the address by min/max, without a branch
+    auto Wsym = [&](int i, int k) -> T {
+        const int hi_ = max(i, k), lo_ = min(i, k);
+        return win[(lo_ & (NSLOT - 1)) * WLD + (hi_ - lo_)];
+    };
     // copy packed columns [c0, c1] between global memory and their ring slots
     auto load_cols = [&](int c0, int c1) {
         if (c1 < c0) return;
@@ -81,7 +107,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         const T *src = Bp + (i64)c0 * ldb;
         for (int idx = tid; idx < cnt; idx += kBCThreads) {
             const int jj = idx / ldb, d = idx - jj * ldb;
-            win[((c0 + jj) & (NSLOT - 1)) * WLD + d] = src[idx];
+            win[((c0 + jj) & (NSLOT - 1)) * WLD + d] = ld_agent(src + idx);
         }
     };
     // exactly b new columns: all loads issued before any LDS store (one round trip)
@@ -89,7 +115,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         const T *src = Bp + (i64)c0 * ldb;
         T v[NEW_PER_THREAD];
 #pragma unroll
-        for (int u = 0; u < NEW_PER_THREAD; ++u) v[u] = src[tid + u * kBCThreads];
+        for (int u = 0; u < NEW_PER_THREAD; ++u) v[u] = ld_agent(src + tid + u * kBCThreads);
 #pragma unroll
         for (int u = 0; u < NEW_PER_THREAD; ++u) {
             const int idx = tid + u * kBCThreads;
@@ -113,13 +139,13 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
 #pragma unroll
             for (int u = 0; u < NEW_PER_THREAD; ++u) {
                 const int idx = tid + u * kBCThreads;
-                if (idx < cnt) dst[idx] = v[u];
+                if (idx < cnt) st_agent(dst + idx, v[u]);
             }
             return;
         }
         for (int idx = tid; idx < cnt; idx += kBCThreads) {
             const int jj = idx / ldb, d = idx - jj * ldb;
-            dst[idx] = win[((c0 + jj) & (NSLOT - 1)) * WLD + d];
+            st_agent(dst + idx, win[((c0 + jj) & (NSLOT - 1)) * WLD + d]);
         }
     };
 
@@ -174,7 +200,6 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                                 break;
                             }
                         }
-                        fence_acquire_agent();
                         s_go = go;
                     }
                     __syncthreads();
@@ -212,10 +237,10 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 T *dst = Bp + (i64)cq * ldb;
                 const T *srcw = win + (size_t)(cq & (NSLOT - 1)) * WLD;
                 for (int d = tid; d < ldb; d += kBCThreads)
-                    dst[d] = (d == o) ? (T)s_beta : ((d > o && d < o + L) ? T(0) : srcw[d]);
+                    st_agent(dst + d, (d == o) ? (T)s_beta : ((d > o && d < o + L) ? T(0) : srcw[d]));
             }
             wg_publish_fence();
-            if (tid == 0) flag_store_release(prog + s, cq);
+            if (tid == 0) flag_store_agent(prog + s, cq);
             if (h > 0 && !wait_and_load()) return;
 
             // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow. Thread = (row t, k segment);
@@ -247,7 +272,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                         for (int j = 0; j < FL; ++j) {
                             const int q = kb + ((t + j) & (FL - 1));
                             const int k = r0 + q;
-                            a4[j & 3] += ((i >= k) ? W_(i, k) : W_(k, i)) * sw[q];
+                            a4[j & 3] += Wsym(i, k) * sw[q];
                         }
                         acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
                     } else {
@@ -255,7 +280,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                         for (int j = 0; j < len; ++j) {
                             const int q = kb + qq;
                             const int k = r0 + q;
-                            const T a = (i >= k) ? W_(i, k) : W_(k, i);
+                            const T a = Wsym(i, k);
                             acc += a * sw[q];
                             qq = (qq + 1 == len) ? 0 : qq + 1;
                         }
@@ -382,7 +407,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         // the rest of the last window
         store_cols(held_lo, held_hi);
         wg_publish_fence();
-        if (tid == 0) flag_store_release(prog + s, kDone);
+        if (tid == 0) flag_store_agent(prog + s, kDone);
     }
 }
 

@@ -51,6 +51,9 @@ namespace {
 
 constexpr int kGroupM = 8;
 
+typedef __attribute__((address_space(1))) void gvoid_t; // global
+typedef __attribute__((address_space(3))) void lvoid_t; // LDS
+
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
@@ -197,9 +200,10 @@ template <typename T, int R, int BK, int NT> struct TileRegs {
     }
 };
 
-// C tile BM x BN of C = alpha op(A) op(B) + beta C over the split's K range.
-template <typename T, int BM, int BN, int WM, int WN, int AMODE, int BMODE, int MINB>
-__global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_tile_kernel(const KParams<T> p) {
+// Register-staged path: C tile BM x BN of C = alpha op(A) op(B) + beta C over the
+// split's K range.
+template <typename T, int BM, int BN, int WM, int WN, int AMODE, int BMODE>
+__device__ __forceinline__ void gemm_body_regs(const KParams<T> &p) {
     constexpr int NT = 64 * WM * WN;
     constexpr int E = Mfma<T>::E;
     constexpr int BK = Mfma<T>::BK;
@@ -354,6 +358,243 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_tile_kernel(const KPa
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// LDS-DMA path (FP64, block tiles of at least 128 x 128): interior tiles of aligned
+// operands go straight from global memory into LDS with global_load_lds_dwordx4 (no
+// register staging, no LDS store instructions), the next k-tile's transfer in flight
+// while the current one computes. Layouts:
+//   KC operand (k contiguous): [row][k], 8 chunks of 16 bytes per row of BK = 16
+//     doubles, chunk c of row r at slot c ^ f(r), f(r) = (r ^ (r >> 3)) & 7, so the
+//     fragment reads of 16 consecutive rows are spread over all banks (no padding, so
+//     one DMA instruction fills 1 KB of consecutive LDS);
+//   RC operand (rows contiguous): [k][row] with a row stride of R + 8 doubles, which
+//     puts the k and k+2 reads of one fragment read in different bank halves.
+// Edge tiles, K tails and unaligned operands are loaded per element (bounds-checked)
+// into the same layouts.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int kc_swz(int r) { return (r ^ (r >> 3)) & 7; }
+
+template <int R> struct DmaTile {
+    static constexpr int BK = 16;
+    static constexpr int RSTR = R + 8; // RC layout: doubles per k row
+    template <bool KC> static constexpr int bytes() { return KC ? R * BK * 8 : BK * RSTR * 8; }
+};
+
+// DMA of one interior R x 16 tile; the LDS destination of each instruction is wave-uniform.
+template <int R, int NT>
+__device__ __forceinline__ void dma_tile(bool kc, const double *p, i64 ld, i64 r0, i64 k0, char *s, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = NT / 64;
+    if (kc) {
+        constexpr int NI = R * 8 / 64 / NW; // R rows x 8 chunks, 64 chunks per instruction
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int inst = wave * NI + i;
+            const int pos = inst * 64 + lane;
+            const int row = pos >> 3, c = (pos & 7) ^ kc_swz(row);
+            const double *g = p + (r0 + row) * ld + k0 + 2 * c;
+            __builtin_amdgcn_global_load_lds((gvoid_t *)g, (lvoid_t *)(s + inst * 1024), 16, 0, 0);
+        }
+    } else {
+        constexpr int PER_K = R / 128; // 1 KB instructions per k row
+        constexpr int NI = 16 * PER_K / NW;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int inst = wave * NI + i;
+            const int kk = inst / PER_K, part = inst % PER_K;
+            const double *g = p + (k0 + kk) * ld + r0 + part * 128 + 2 * lane;
+            __builtin_amdgcn_global_load_lds((gvoid_t *)g,
+                                             (lvoid_t *)(s + ((size_t)kk * DmaTile<R>::RSTR + part * 128) * 8), 16,
+                                             0, 0);
+        }
+    }
+}
+
+// Per-element load of an R x 16 tile (rows < rlim, k < klim; zero elsewhere) into the
+// DMA layouts (synchronous: used for edge tiles, K tails and unaligned operands).
+template <int R, int NT>
+__device__ __forceinline__ void scalar_tile(bool kc, const double *p, i64 ld, i64 r0, i64 k0, i64 rlim, i64 klim,
+                                            char *s, int tid) {
+    double *d = reinterpret_cast<double *>(s);
+    constexpr int PER = R * 16 / NT;
+    // one element at a time (edge tiles only): few live registers beside the accumulators
+#pragma unroll 1
+    for (int q = 0; q < PER; ++q) {
+        const int idx = tid + q * NT;
+        int row, kk;
+        if (kc) {
+            row = idx >> 4;
+            kk = idx & 15;
+        } else {
+            row = idx % R;
+            kk = idx / R;
+        }
+        const i64 r = r0 + row, k = k0 + kk;
+        const double v = (r < rlim && k < klim) ? (kc ? p[k + r * ld] : p[r + k * ld]) : 0.0;
+        if (kc) d[(row * 8 + ((kk >> 1) ^ kc_swz(row))) * 2 + (kk & 1)] = v;
+        else d[kk * DmaTile<R>::RSTR + row] = v;
+    }
+}
+
+// fragment read: lane (i, j) gets rows rb + i at k = 8 kg + 2j + {0, 1}
+template <int R>
+__device__ __forceinline__ Mfma<double>::vec dma_frag(bool kc, const char *s, int rb, int kg, int li, int lj) {
+    if (kc) {
+        const int row = rb + li, c = kg * 4 + lj;
+        return *reinterpret_cast<const Mfma<double>::vec *>(s + (row * 8 + (c ^ kc_swz(row))) * 16);
+    }
+    const double *d = reinterpret_cast<const double *>(s);
+    const int k = kg * 8 + 2 * lj;
+    Mfma<double>::vec v;
+    v[0] = d[k * DmaTile<R>::RSTR + rb + li];
+    v[1] = d[(k + 1) * DmaTile<R>::RSTR + rb + li];
+    return v;
+}
+
+template <int BM, int BN, int WM, int WN, int AMODE, int BMODE>
+__device__ __forceinline__ void gemm_body_dma(const KParams<double> &p) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BK = 16;
+    constexpr int TM = BM / WM, TN = BN / WN;
+    constexpr int FM = TM / 16, FN = TN / 16;
+    constexpr bool AKC = AMODE == A_T, BKC = BMODE == B_N;
+    constexpr int ABYTES = DmaTile<BM>::template bytes<AKC>();
+    constexpr int BBYTES = DmaTile<BN>::template bytes<BKC>();
+    constexpr int STAGE = ABYTES + BBYTES;
+    using V4 = Mfma<double>::v4;
+    using vec = Mfma<double>::vec;
+    __shared__ __attribute__((aligned(1024))) char smem[2 * STAGE];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int nbk = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, q8 = nbk >> 3, rmd = nbk & 7;
+    const int t = xcd * q8 + min(xcd, rmd) + (bid >> 3);
+    int tm, tn;
+    tile_coords(t, p.tiles_m, p.tiles_n, p.lower, tm, tn);
+    const i64 m0 = (i64)tm * BM, n0 = (i64)tn * BN;
+    const i64 kbeg = (i64)blockIdx.y * p.kchunk;
+    const i64 kend = min(p.K, kbeg + p.kchunk);
+    const int nkt = (int)((kend - kbeg + BK - 1) / BK);
+    const bool m_in = m0 + BM <= p.M, n_in = n0 + BN <= p.N;
+
+    V4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+    // issue (or, per element, complete) the transfer of k-tile kt into its stage
+    auto issue = [&](int kt) {
+        char *st = smem + (kt & 1) * STAGE;
+        const i64 k0 = kbeg + (i64)kt * BK;
+        const bool k_in = k0 + BK <= kend;
+        {
+            const double *pa = p.a0;
+            i64 lda = p.lda0, ko = 0;
+            if constexpr (AMODE == A_CAT) {
+                if (k0 >= p.asplit) {
+                    pa = p.a1;
+                    lda = p.lda1;
+                    ko = p.asplit;
+                }
+            }
+            if (m_in && k_in && p.avec) dma_tile<BM, NT>(AKC, pa, lda, m0, k0 - ko, st, tid);
+            else scalar_tile<BM, NT>(AKC, pa, lda, m0, k0 - ko, p.M, kend - ko, st, tid);
+        }
+        {
+            const double *pb = p.b0;
+            i64 ldb = p.ldb0, ko = 0;
+            if constexpr (BMODE == B_CATT) {
+                if (k0 >= p.bsplit) {
+                    pb = p.b1;
+                    ldb = p.ldb1;
+                    ko = p.bsplit;
+                }
+            }
+            if (n_in && k_in && p.bvec) dma_tile<BN, NT>(BKC, pb, ldb, n0, k0 - ko, st + ABYTES, tid);
+            else scalar_tile<BN, NT>(BKC, pb, ldb, n0, k0 - ko, p.N, kend - ko, st + ABYTES, tid);
+        }
+    };
+
+    const int li = lane & 15, lj = lane >> 4;
+    if (nkt > 0) issue(0);
+    for (int kt = 0; kt < nkt; ++kt) {
+        // this wave's transfers of tile kt are complete; the barrier makes every wave's
+        // visible and ends every wave's reads of tile kt - 1, whose stage kt + 1 reuses
+        __builtin_amdgcn_s_waitcnt(0x0070); // vmcnt(0) lgkmcnt(0)
+        __syncthreads();
+        if (kt + 1 < nkt) issue(kt + 1);
+        const char *st = smem + (kt & 1) * STAGE;
+#pragma unroll
+        for (int kg = 0; kg < BK / 8; ++kg) {
+            vec af[FM], bf[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = dma_frag<BM>(AKC, st, wm * TM + i * 16, kg, li, lj);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = dma_frag<BN>(BKC, st + ABYTES, wn * TN + j * 16, kg, li, lj);
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<double>::mma(bf[j][e], af[i][e], acc[i][j]);
+            // spread this group's fragment reads between its MFMAs
+#pragma unroll
+            for (int u = 0; u < FM + FN; ++u) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 2 * FM * FN - 2 * (FM + FN), 0);
+        }
+    }
+
+    // ---- epilogue
+    const bool partial = p.part != nullptr;
+    double *part = partial ? p.part + (i64)blockIdx.y * p.M * p.N : nullptr;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const i64 m = m0 + wm * TM + i * 16 + li;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 n = n0 + wn * TN + j * 16 + Mfma<double>::out_row(lane, r);
+                if (m >= p.M || n >= p.N) continue;
+                if (p.lower && m < n) continue;
+                const double v = acc[i][j][r];
+                if (partial) {
+                    part[m + n * p.M] = v;
+                } else {
+                    double *cp = p.C + m + n * p.ldc;
+                    *cp = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * (*cp);
+                }
+            }
+        }
+    }
+}
+
+// The DMA path serves FP64 tiles of at least 128 x 128 with plain or K-concatenated
+// operands when both operands allow 16-byte transfers; symmetric A, unaligned
+// operands and everything else take the register-staged path, which overlaps its
+// per-element loads with the computation.
+template <typename T, int BM, int BN, int AMODE>
+constexpr bool dma_capable() {
+    return sizeof(T) == 8 && BM >= 128 && BN >= 128 && AMODE != A_SYM;
+}
+
+template <typename T, int BM, int BN, int WM, int WN, int AMODE, int BMODE, int MINB, bool DMA>
+__global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_tile_kernel(const KParams<T> p) {
+    if constexpr (DMA && dma_capable<T, BM, BN, AMODE>()) {
+        gemm_body_dma<BM, BN, WM, WN, AMODE, BMODE>(p);
+    } else {
+        gemm_body_regs<T, BM, BN, WM, WN, AMODE, BMODE>(p);
+    }
+}
+
 // C = alpha * sum_s part[s] + beta * C (fixed summation order).
 template <typename T>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int splits, i64 M,
@@ -375,7 +616,16 @@ enum Cfg { C128x128 = 0, C128x64 = 1, C64x128 = 2, C64x64 = 3, C256x128 = 4 };
 template <typename T, int BM, int BN, int WM, int WN, int MINB, int AM, int BMD>
 int launch_one(hipStream_t s, const KParams<T> &p, int splits, int ntiles) {
     dim3 grid((unsigned)ntiles, (unsigned)splits);
-    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, WM, WN, AM, BMD, MINB>), grid, dim3(64 * WM * WN), 0, s, p);
+    if constexpr (dma_capable<T, BM, BN, AM>()) {
+        if (p.avec && p.bvec) {
+            hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, WM, WN, AM, BMD, MINB, true>), grid, dim3(64 * WM * WN),
+                               0, s, p);
+            CUEV_LAUNCH_CHECK();
+            return CUEV_STATUS_SUCCESS;
+        }
+    }
+    hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, WM, WN, AM, BMD, MINB, false>), grid, dim3(64 * WM * WN), 0, s,
+                       p);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
 }

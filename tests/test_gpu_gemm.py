@@ -88,3 +88,37 @@ def test_syr2k_lower(torch_dev, n, k):
     up = np.triu_indices(n, 1)
     assert np.abs(got[low] - ref[low]).max() / (np.abs(ref).max()) < 1e-13
     assert np.array_equal(got[up], C0[up]), "upper triangle must be untouched"
+
+
+@pytest.mark.parametrize("m,n,k,ta,tb,pad", [
+    (1000, 700, 999, 0, 0, 0),     # block tiles of 128 x 128: interior DMA tiles, edge tiles, K tail
+    (520, 390, 700, 1, 0, 0),
+    (2048, 1536, 512, 0, 1, 0),
+    (700, 1100, 300, 1, 1, 0),
+    (1024, 1024, 1024, 1, 0, 1),   # odd leading dimensions: the per-element path everywhere
+    (777, 1024, 640, 0, 0, 2),     # even padding: aligned again
+])
+def test_gemm_large_tiles(torch_dev, m, n, k, ta, tb, pad):
+    """FP64 products large enough for the 128 x 128 block tiles (the LDS-DMA path),
+    with edges, K tails and leading dimensions larger than the matrices (odd ones
+    force the per-element loads)."""
+    torch = torch_dev
+    rng = np.random.default_rng(m + 3 * n + 7 * k + pad)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    C0 = rng.standard_normal((m, n))
+    lda, ldb, ldc = A.shape[0] + pad, B.shape[0] + pad, m + pad
+    def padded(X, ld):
+        Y = np.zeros((ld, X.shape[1]))
+        Y[: X.shape[0]] = X
+        return to_dev(Y)
+    dA, dB, dC = padded(A, lda), padded(B, ldb), padded(C0, ldc)
+    alpha, beta = -0.75, 1.5
+    cv.gemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
+    torch.cuda.synchronize()
+    ref = alpha * ((A.T if ta else A) @ (B.T if tb else B)) + beta * C0
+    got = from_dev(dC)[:m]
+    scale = np.abs(A).max() * np.abs(B).max() * k + np.abs(C0).max()
+    assert np.abs(got - ref).max() / scale < 4e-14 * np.sqrt(k)
+    if pad:
+        assert np.all(from_dev(dC)[m:] == 0), "rows beyond m of C must be untouched"
