@@ -49,6 +49,18 @@ namespace {
 
 constexpr int kBCThreads = 512; // 8 waves: two per SIMD, so LDS latency overlaps
 constexpr int kDone = 0x3fffffff; // prog value of a finished sweep
+constexpr int kMaxPhases = 8;
+
+// Sweep schedule in phases: sweeps [s0[p], s0[p+1]) go round-robin to workgroups
+// 0 .. g[p]-1 (g non-increasing). A sweep of h hops is active for ~h hop times and a
+// new one starts every ~2 hop times, so fewer sweeps are in flight as they shorten;
+// a workgroup with no sweep left in the later phases exits and frees its CU for the
+// back-transform products running beside the chase.
+struct ChasePhases {
+    int np;
+    int s0[kMaxPhases + 1];
+    int g[kMaxPhases];
+};
 
 // Band traffic between workgroups. Every global load and store of the packed band is
 // an agent-scope relaxed atomic access: on gfx950 these carry the sc1 bit, so a store
@@ -72,7 +84,7 @@ __device__ __forceinline__ void flag_store_agent(int *p, int v) {
 
 template <typename T, int B>
 __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 ldu, int n,
-                                                              int *prog, int *err) {
+                                                              int *prog, int *err, const ChasePhases ph) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int b = B;
     constexpr int NEW_PER_THREAD = 2 * B * B / kBCThreads; // b new columns of 2b rows per hop
@@ -167,7 +179,18 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         if (lane == 0) s_beta = beta;
     };
 
-    for (int s = blockIdx.x; s < n - 2; s += gridDim.x) {
+    // this workgroup's sweeps in increasing order: the next one of the current phase,
+    // else its first one in a later phase that still uses it (none left: n)
+    int phase = 0;
+    auto settle = [&](int s_) {
+        while (s_ >= ph.s0[phase + 1]) {
+            if (++phase >= ph.np) return n;
+            s_ = ((int)blockIdx.x < ph.g[phase]) ? ph.s0[phase] + (int)blockIdx.x : ph.s0[phase + 1];
+        }
+        return s_;
+    };
+    int s = settle(((int)blockIdx.x < ph.g[0]) ? ph.s0[0] + (int)blockIdx.x : ph.s0[1]);
+    for (; s < n - 2;) {
         int held_lo = 0, held_hi = -1; // window columns currently held in LDS
         bool hh_ready = false;         // this hop's reflector was formed at the end of the last
         for (int h = 0;; ++h) {
@@ -408,6 +431,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         store_cols(held_lo, held_hi);
         wg_publish_fence();
         if (tid == 0) flag_store_agent(prog + s, kDone);
+        s = settle(s + ph.g[phase]);
     }
 }
 
@@ -446,12 +470,23 @@ template <typename T, int B> int launch_chase(Workspace<T> &ws, T *Bp) {
     // 84: 1.215 s)
     const int useful = (int)ceil_div((i64)n * 13, 32 * B) + 8;
     const int grid = std::max(1, std::min({nsweeps, resident, useful}));
+    // phases of equal sweep counts; phase p keeps the workgroups its sweeps' lengths
+    // call for, by the same rule applied to the n - s0 columns left
+    ChasePhases ph{};
+    ph.np = std::min(kMaxPhases, std::max(1, nsweeps / 256));
+    for (int q = 0; q <= ph.np; ++q) ph.s0[q] = (int)((i64)nsweeps * q / ph.np);
+    for (int q = 0; q < ph.np; ++q) {
+        const int left = n - ph.s0[q];
+        const int want = (int)ceil_div((i64)left * 13, 32 * B) + 8;
+        ph.g[q] = std::max(1, std::min(grid, want));
+        if (q > 0) ph.g[q] = std::min(ph.g[q], ph.g[q - 1]);
+    }
     // algorithmic work: ~n^2/(2b) hops, each reading and writing b columns of the
     // packed band (2 * 2b^2 S bytes) and doing ~ (2 * 3b * b + 4 * 4b^2) flops
     const double hops = (double)n * n / (2.0 * B);
     prof::Scope sc(s, "bc.chase", hops * (6.0 * B * B + 16.0 * B * B), hops * 4.0 * B * B * sizeof(T));
     hipLaunchKernelGGL((bc_chase_kernel<T, B>), dim3(grid), dim3(kBCThreads), lds, s, Bp, ws.U,
-                       ws.ldu, n, ws.prog, ws.err);
+                       ws.ldu, n, ws.prog, ws.err, ph);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;
 }
