@@ -826,14 +826,25 @@ int sbr_back_t(Workspace<T> &ws, hipStream_t s, const T *Y, const T *W, T *M, i6
     const i64 ld = ws.ld;
     if (ldm < n) CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "sbr_back_t: ldm < n");
     Scratch gs{ws.gscratch, ws.gscratch_bytes};
-    // Q_s^T = ... (I - Yhat_1 What_1^T)(I - Yhat_0 What_0^T): blocks first to last,
-    // M -= Yhat (What^T M) with Yhat the block's Y columns, zero above each panel's
-    // rows (copied into ws.Yh, since Y there holds nothing defined)
-    for (int i = 0; i + b < n; i += k) {
+    // M = Q_s^T = Q_T^T ... Q_1^T Q_0^T with Q_t^T = I - Yhat_t What_t^T for the
+    // reduction's blocks t (Yhat the block's Y columns, zero above each panel's rows,
+    // copied into ws.Yh since Y there holds nothing defined). Built from the right,
+    // last block first: M = I, then M <- M Q_t^T = M - (M Yhat_t) What_t^T for
+    // t = T .. 0. Before block t, M differs from I only in its trailing block from
+    // row and column r0_t + k on, and Yhat_t, What_t vanish above row r0_t, so the
+    // update touches the trailing (n - r0_t)^2 block alone:
+    //   P = M[r0:, r0:] Yhat_t[r0:],   M[r0:, r0:] -= P What_t[r0:]^T
+    // (4/3 n^3 flops in all, against 2 n^3 for applying the blocks from the left to
+    // every column).
+    CUEV_TRY(fill2d<T>(s, M, ldm, n, n, T(0), T(1)));
+    int nblk = 0;
+    for (int i = 0; i + b < n; i += k) ++nblk;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int i = bi * k;
         int np = 0;
         while (np * b < k && i + np * b + b < n) ++np;
         const int r0 = i + b;
-        const i64 R0 = n - r0;
+        const i64 R0 = n - r0, kb = (i64)np * b;
         {
             prof::Scope sc(s, "bt.sbr_what");
             CUEV_TRY(build_what<T>(ws, s, Y, W, i, np, gs));
@@ -844,10 +855,11 @@ int sbr_back_t(Workspace<T> &ws, hipStream_t s, const T *Y, const T *W, T *M, i6
                 CUEV_TRY(copy2d<T>(s, Y + (i64)j * ld + rt, ld, Yt + rt, ld, n - rt, b));
             }
         }
-        prof::Scope sc(s, "bt.sbr_back", 4.0 * (double)R0 * np * b * n);
-        CUEV_TRY(gemm<T>(s, 1, 0, (i64)np * b, n, R0, T(1), ws.Z + r0, ld, M + r0, ldm, T(0), ws.Zt, k, gs));
-        CUEV_TRY(gemm<T>(s, 0, 0, R0, n, (i64)np * b, T(-1), ws.Yh + r0, ld, ws.Zt, k, T(1), M + r0, ldm,
-                         gs));
+        prof::Scope sc(s, "bt.sbr_back", 4.0 * (double)R0 * R0 * kb);
+        T *Mt = M + r0 + (i64)r0 * ldm;
+        // P (R0 x kb, ld n) in ws.Zt, which holds k x n elements
+        CUEV_TRY(gemm<T>(s, 0, 0, R0, kb, R0, T(1), Mt, ldm, ws.Yh + r0, ld, T(0), ws.Zt, n, gs));
+        CUEV_TRY(gemm<T>(s, 0, 1, R0, R0, kb, T(-1), ws.Zt, n, ws.Z + r0, ld, T(1), Mt, ldm, gs));
     }
     return CUEV_STATUS_SUCCESS;
 }

@@ -9,9 +9,9 @@ namespace cuev {
 template <typename T> int bc_back(Workspace<T> &ws, const T *U, T *M, i64 ldm);
 // M <- Q_s M for an n x n M (ld ldm >= n); Y, W with leading dimension ws.ld.
 template <typename T> int sbr_back(Workspace<T> &ws, const T *Y, const T *W, T *M, i64 ldm);
-// Transposes, on stream s: M <- Q_b^T M and M <- Q_s^T M. With them X^T = Q_b^T Q_s^T
-// can be built in the order Q_s^T first (it needs only the band reduction, so it can
-// run while the bulge chasing runs), then Q_b^T.
+// On stream s: M <- Q_b^T M, and M <- Q_s^T (the input M is not read). With them
+// X^T = Q_b^T Q_s^T is built in the order Q_s^T first (it needs only the band
+// reduction, so it can run while the bulge chasing runs), then Q_b^T.
 template <typename T> int bc_back_t(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm);
 template <typename T>
 int sbr_back_t(Workspace<T> &ws, hipStream_t s, const T *Y, const T *W, T *M, i64 ldm);

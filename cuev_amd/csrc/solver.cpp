@@ -137,7 +137,6 @@ int solve_native(Context<T> &ctx, T *A, T *eval, T *evec, cuev_solve_timer *time
         // CUs, LDS-bound); Q_b^T follows the chasing and runs beside the D&C.
         CUEV_HIP(hipStreamWaitEvent(bs, ev[1], 0));
         CUEV_HIP(hipEventRecord(ctx.xev[0], bs));
-        CUEV_TRY(fill2d<T>(bs, ws.M, ws.ld, n, n, T(0), T(1)));
         CUEV_TRY(sbr_back_t<T>(ws, bs, ws.Y, ws.W, ws.M, ws.ld));
         CUEV_HIP(hipEventRecord(ctx.xev[1], bs));
         CUEV_HIP(hipStreamWaitEvent(bs, ev[2], 0));

@@ -222,3 +222,54 @@ def test_bc_back_block_schedule_equals_oracle(n, b):
     assert np.abs(_bc_back_blocked(r["U"], M0, b) - ref).max() < 1e-13
     # and the check has teeth: the forward (transposed) order differs
     assert np.abs(O.bc_back(r["U"], M0, b, reverse_order=False) - ref).max() > 1e-3
+
+
+def _sbr_back_t_right(Y, W, b, k):
+    """Q_s^T as the device's sbr_back_t builds it, in numpy: the reduction's blocks in
+    compact form Q_t = I - What_t Yhat_t^T (What built left to right,
+    What_t = W_t - What_{0:t} (Yhat_{0:t}^T W_t)), then M = I and, last block first,
+    M[r0:, r0:] -= (M[r0:, r0:] Yhat[r0:]) What[r0:]^T (right multiplication by Q_t^T
+    restricted to the trailing block)."""
+    n = Y.shape[0]
+
+    def panel(j):
+        rt = j + b
+        Yp = np.zeros((n, b))
+        Wp = np.zeros((n, b))
+        Yp[rt:] = Y[rt:, j:j + b]
+        Wp[rt:] = W[rt:, j:j + b]
+        return Yp, Wp
+
+    blocks = []
+    i = 0
+    while i + b < n:
+        Yh, Wh = [], []
+        t = 0
+        while t * b < k and i + t * b + b < n:
+            Yp, Wp = panel(i + t * b)
+            if Wh:
+                Wp = Wp - np.hstack(Wh) @ (np.hstack(Yh).T @ Wp)
+            Yh.append(Yp)
+            Wh.append(Wp)
+            t += 1
+        blocks.append((i, np.hstack(Yh), np.hstack(Wh)))
+        i += k
+    M = np.eye(n)
+    for i, Yh, Wh in reversed(blocks):
+        r0 = i + b
+        P = M[r0:, r0:] @ Yh[r0:]
+        M[r0:, r0:] -= P @ Wh[r0:].T
+    return M
+
+
+@pytest.mark.parametrize("n,b,k", [(300, 16, 64), (257, 32, 96), (500, 64, 128)])
+def test_sbr_back_t_right_application_equals_oracle(n, b, k):
+    """The trailing-block right application the device uses for Q_s^T equals the
+    transpose of the oracle's sbr_back (the reference's panel order) applied to I."""
+    A0 = O.fill_glibc_matrix(n, n, n)
+    r = O.dbbr_reduce(A0, b, k)
+    Qs = O.sbr_back(r["Y"], r["W"], np.eye(n, order="F"), b)
+    got = _sbr_back_t_right(r["Y"], r["W"], b, k)
+    assert np.abs(got - Qs.T).max() < 1e-13
+    # and the check has teeth: Q_s itself (no transpose) differs
+    assert np.abs(got - Qs).max() > 1e-3
