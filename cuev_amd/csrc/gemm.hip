@@ -604,8 +604,21 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int s
     if (idx >= M * N) return;
     const i64 m = idx % M, n = idx / M;
     if (lower && m < n) return;
-    T s = T(0);
-    for (int k = 0; k < splits; ++k) s += part[(i64)k * M * N + idx];
+    // four interleaved partial sums keep eight loads in flight per thread; the
+    // order of the additions is fixed, so every run rounds alike
+    const i64 MN = M * N;
+    const T *q = part + idx;
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    int k = 0;
+#pragma unroll 2
+    for (; k + 4 <= splits; k += 4) {
+        s0 += q[(i64)k * MN];
+        s1 += q[(i64)(k + 1) * MN];
+        s2 += q[(i64)(k + 2) * MN];
+        s3 += q[(i64)(k + 3) * MN];
+    }
+    for (; k < splits; ++k) s0 += q[(i64)k * MN];
+    const T s = (s0 + s1) + (s2 + s3);
     T *c = C + m + n * ldc;
     *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
 }
@@ -714,7 +727,10 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     };
     int cfg;
     const bool sym_panel = A.sym && N <= 64;
-    if (scratch.small_tiles) {
+    // a tall-skinny update (the band reduction's panel-wide products): 64 x 64 tiles
+    // for the row parallelism
+    const bool tall_skinny = !lower && !A.sym && N <= 64 && M >= 2048 && K < 1024;
+    if (tall_skinny || scratch.small_tiles) {
         cfg = C64x64;
     } else if (lower) {
         cfg = (N >= 1024) ? C128x128 : C64x64;
@@ -736,12 +752,19 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     p.tiles_m = (int)ceil_div(M, BMs);
     p.tiles_n = (int)ceil_div(N, BNs);
 
-    // split-K for products with few output tiles and a long K
+    // split-K for products with few output tiles and a long K. The K chunks are
+    // short (64): a workgroup of these products waits on memory latency at every
+    // K step, so many short chunks keep more loads in flight than a few long ones.
+    // Tall-skinny updates (N <= 64, short K) are split four ways.
     int splits = 1;
-    const i64 target = (sym_panel ? 8LL : 2LL) * cus;
-    const i64 kmin = sym_panel ? 256 : 512; // shortest K chunk per split
+    const i64 target = (sym_panel ? 16LL : 2LL) * cus;
+    const i64 kmin = sym_panel ? 256 : 64; // shortest K chunk per split
     if (K == 0) {
         splits = 0;
+    } else if (tall_skinny && K >= 128) {
+        const i64 cap = (i64)(scratch.bytes / ((size_t)M * N * sizeof(T)));
+        const i64 want = std::min<i64>(std::min<i64>(4, K / 64), cap);
+        if (want > 1) splits = (int)want;
     } else if (ntiles < target / 2 && K >= 1024) {
         i64 want = std::min<i64>(ceil_div(target, ntiles), K / kmin);
         const i64 cap = (i64)(scratch.bytes / ((size_t)M * N * sizeof(T)));
