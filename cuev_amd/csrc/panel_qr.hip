@@ -46,6 +46,7 @@ template <typename T> struct PQArgs {
     double *pivot;  // [2][b]
     const int *gate; // when set: run only if *gate != 0 (the CholQR kernel declined the panel)
     unsigned *counter;
+    unsigned *reset; // the CholQR kernel's barrier counter, zeroed for the next panel
     int *err;
 };
 
@@ -73,8 +74,48 @@ __device__ __forceinline__ bool grid_barrier(unsigned *counter, unsigned target,
     return s_ok != 0;
 }
 
+// Write-through exchange for the CholQR kernel: every global access to data that
+// another workgroup of the same launch reads (Gram partials, their sums, U^-1 and T)
+// is an agent-scope relaxed atomic access. On gfx950 these carry the sc1 bit: the
+// store is written through past the XCD-private L2 and the load reads from the
+// device-coherent level. A grid barrier then needs neither an L2 write-back before
+// the arrival nor an L2 invalidate after the wait: each wave waits until its own
+// stores have completed (vmcnt(0)), the workgroup meets, one thread increments the
+// counter and polls it, and the workgroup meets again.
+__device__ __forceinline__ double ld_wt(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_wt(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool grid_barrier_wt(unsigned *counter, unsigned target, int *err) {
+    vm_drain();
+    __syncthreads();
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long long it = 0;
+        int ok = 1;
+        while (counter_load_relaxed(counter) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++it > kSpinLimit) {
+                __hip_atomic_store(err, (int)DEV_WATCHDOG, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
 template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_kernel(PQArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    // the CholQR kernel before this one has finished: its counter is free
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(a.reset, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (a.gate && *a.gate == 0) return;
     const int b = a.b, CH = a.CH, CHp = CH + 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -336,6 +377,7 @@ template <typename T> struct CQArgs {
     double *bc;   // [2*b*b]: U^-1, then T (column-major)
     int *gate;
     unsigned *counter;
+    unsigned *reset; // the Householder kernel's barrier counter, zeroed for the launch after this one
     int *err;
 };
 
@@ -372,7 +414,7 @@ __device__ __forceinline__ void cq_gram(const double *P, int CHp, int nr, double
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = ti + NB * u, j = tj + NB * v;
-            if (i >= j) out[cq_lo<B>(i, j)] = acc[u][v];
+            if (i >= j) st_wt(out + cq_lo<B>(i, j), acc[u][v]);
         }
 }
 
@@ -465,6 +507,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
     double *sv = gd + B;      // [B] signs S
     double *red = sv + B;     // [kCQThreads]
 
+    if (g == 0 && tid == 0) __hip_atomic_store(a.reset, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int r0g = g * CH;
     const int chl = max(0, min(CH, a.rows - r0g));
     for (int idx = tid; idx < B * CH; idx += kCQThreads) {
@@ -477,7 +520,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
     for (int pass = 0; pass < 2; ++pass) {
         // ---- Gram matrix of the panel: partials, then fixed-order slice sums
         cq_gram<B>(P, CHp, chl, a.part + (size_t)g * BL, tid);
-        if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
+        if (!grid_barrier_wt(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
         {
             const int S = (BL + a.G - 1) / a.G;
             const int e0 = g * S, ne = max(0, min(BL, e0 + S) - e0);
@@ -494,7 +537,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
 #pragma unroll
                         for (int u = 0; u < 16; ++u) {
                             const int gg = g0 + u * tpe;
-                            v[u] = (gg < a.G) ? src[(size_t)gg * BL] : 0.0;
+                            v[u] = (gg < a.G) ? ld_wt(src + (size_t)gg * BL) : 0.0;
                         }
 #pragma unroll
                         for (int u = 0; u < 16; ++u) s += v[u];
@@ -505,12 +548,12 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 if (el < ne && pt == 0) {
                     double t = 0.0;
                     for (int q = 0; q < tpe; ++q) t += red[tid + q];
-                    a.gsum[e0 + el] = t;
+                    st_wt(a.gsum + e0 + el, t);
                 }
                 __syncthreads();
             }
         }
-        if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
+        if (!grid_barrier_wt(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
 
         // ---- second pass, G = I + E with E tiny (the usual case): R_2 = I + F,
         // F = up(E - up(E)^T up(E)) with up(M) = strict upper of M + diag(M)/2,
@@ -523,7 +566,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int i = si + NB * u, j = sj + NB * v;
-                    const double gij = own ? a.gsum[(i >= j) ? cq_lo<B>(i, j) : cq_lo<B>(j, i)] : 0.0;
+                    const double gij = own ? ld_wt(a.gsum + ((i >= j) ? cq_lo<B>(i, j) : cq_lo<B>(j, i))) : 0.0;
                     e[u][v] = own ? gij - (i == j ? 1.0 : 0.0) : 0.0;
                     emax = fmax(emax, fabs(e[u][v]));
                     if (own) M2[i + j * LD] = (i < j) ? e[u][v] : (i == j ? 0.5 * e[u][v] : 0.0);
@@ -587,11 +630,11 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = 4 * ti + u, j = 4 * tj + v;
-                gr[u][v] = own ? a.gsum[(i >= j) ? cq_lo<B>(i, j) : cq_lo<B>(j, i)] : 0.0;
+                gr[u][v] = own ? ld_wt(a.gsum + ((i >= j) ? cq_lo<B>(i, j) : cq_lo<B>(j, i))) : 0.0;
                 xr[u][v] = (i == j) ? 1.0 : 0.0;
             }
         double *Rout = (pass == 0) ? M1 : M3;
-        if (tid < B) gd[tid] = a.gsum[cq_lo<B>(tid, tid)];
+        if (tid < B) gd[tid] = ld_wt(a.gsum + cq_lo<B>(tid, tid));
         if (own && tj == 0)
 #pragma unroll
             for (int u = 0; u < 4; ++u) cb[4 * ti + u] = gr[u][0];
@@ -838,9 +881,9 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 for (int v = 0; v < 4; ++v) {
                     const int i = 4 * ti + u, j = 4 * tj + v;
                     M3[i + j * LD] = vr[u][v];
-                    a.bc[i + (size_t)j * B] = vr[u][v];
+                    st_wt(a.bc + i + (size_t)j * B, vr[u][v]);
                 }
-        for (int idx = tid; idx < BB; idx += kCQThreads) a.bc[BB + idx] = M1[(idx % B) + (idx / B) * LD];
+        for (int idx = tid; idx < BB; idx += kCQThreads) st_wt(a.bc + BB + idx, M1[(idx % B) + (idx / B) * LD]);
         // Y rows 0..b-1 into P (unit lower L)
         for (int idx = tid; idx < BB; idx += kCQThreads) {
             const int r = idx % B, j = idx / B;
@@ -848,12 +891,12 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
         }
         if (tid == 0) *a.gate = 0;
     }
-    if (!grid_barrier(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
+    if (!grid_barrier_wt(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
     if (g != 0) {
         for (int idx = tid; idx < BB; idx += kCQThreads) {
             const int i = idx % B, j = idx / B;
-            M3[i + j * LD] = a.bc[idx];
-            M1[i + j * LD] = a.bc[BB + idx];
+            M3[i + j * LD] = ld_wt(a.bc + idx);
+            M1[i + j * LD] = ld_wt(a.bc + BB + idx);
         }
     }
     __syncthreads();
@@ -977,6 +1020,7 @@ int launch_cholqr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T 
     a.gsum = a.part + (size_t)G * B * B;
     a.bc = a.gsum + B * B;
     a.counter = counter;
+    a.reset = counter + 1;
     a.err = err;
     hipLaunchKernelGGL((panel_cholqr_kernel<T, B>), dim3(G), dim3(kCQThreads), lds, s, a);
     CUEV_LAUNCH_CHECK();
@@ -993,9 +1037,11 @@ int panel_qr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T *Tm, 
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "panel_qr: rows=%d b=%d", rows, b);
     if (scratch_bytes < kPQGateBytes)
         CUEV_FAIL(CUEV_STATUS_INVALID_ARGUMENT, "panel_qr: scratch too small");
-    // counter[0]: CholQR barriers, counter[1]: Householder barriers
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, counter, 2);
-    CUEV_LAUNCH_CHECK();
+    // counter[0]: CholQR barriers, counter[1]: Householder barriers. Both are zero
+    // when a panel starts: the workspace starts them at zero, each kernel zeroes the
+    // other's counter when it starts (the CholQR kernel the Householder kernel's that
+    // follows it, the Householder kernel, always launched, the CholQR counter for the
+    // next panel), and without a CholQR launch counter[1] is zeroed here.
     bool cq = false;
     if (rows >= 2 * b) {
         if (b == 64)
@@ -1043,7 +1089,12 @@ int panel_qr(hipStream_t s, T *A, i64 lda, T *Y, i64 ldy, T *W, i64 ldw, T *Tm, 
     a.pivot = a.part + (size_t)2 * G * 2 * b;
     a.gate = cq ? static_cast<const int *>(scratch) : nullptr;
     a.counter = counter + 1;
+    a.reset = counter;
     a.err = err;
+    if (!cq) {
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, counter + 1, 1);
+        CUEV_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(panel_qr_kernel<T>, dim3(G), dim3(kPQThreads), lds, s, a);
     CUEV_LAUNCH_CHECK();
     return CUEV_STATUS_SUCCESS;

@@ -124,6 +124,7 @@ template <typename T> int workspace_alloc(Workspace<T> &ws, int n, int b, int k,
     // U slack beyond ld*n is read (masked) by bc_back; give it defined contents.
     CUEV_TRY(fill2d<T>(s, ws.U, (i64)nu, (i64)nu, 1, T(0), T(0)));
     CUEV_TRY(fill_int(s, ws.err, 1, 0));
+    CUEV_TRY(fill_int(s, reinterpret_cast<int *>(ws.counter), 2, 0)); // panel QR barrier counters
     return CUEV_STATUS_SUCCESS;
 }
 
