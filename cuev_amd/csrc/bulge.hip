@@ -20,17 +20,20 @@
 //    finished with (a ring of 2b column slots). The Householder vector is formed
 //    by wave 0 alone (L <= 64 = one wave64), with FP64 norms.
 //  * Dependency: hop (s, cq, r1) touches packed columns [cq, r1] only, and the
-//    previous sweep's hops touch columns [cq'', r1''] with cq'' increasing. Of
-//    the hop of sweep s-1 with cq'' = r1, hop (s, cq) shares column r1 alone, and
-//    that column is final (beta e_0 below the pivot) as soon as its reflector is
-//    formed. So each hop writes its column cq back and publishes progress
-//    prog[s] = cq right after the Householder step; sweep s starts the hop once
-//    prog[s-1] >= r1, overlapping the rest of that hop of sweep s-1 (which then
-//    touches column r1 only in its own LDS). After the first hop of a sweep the
-//    reflector needs only column cq, already in LDS and final with respect to
-//    sweep s-1, so a hop runs: reflector, write-back + flag, wait for sweep s-1,
-//    load the b new columns, rank-2 update. The reference waits for a margin of
-//    3b columns (bulge.cu:193-199); the critical path is ~ n times the lag.
+//    previous sweep's hops touch columns [cq'', r1''] with cq'' increasing. Hop h of
+//    sweep s reads columns up to r1 = r0 + b - 1, which sweep s-1 last changes in its
+//    hop h+1 (the rank-2 update of that hop's R1 columns and of its column r0'' = r1,
+//    and the elimination of column r1 by its hop h+2's reflector). So a hop publishes
+//    progress prog[s] = r0 early: after R1 and column r0 are updated, the next hop's
+//    reflector is formed from column r0, and R1's columns and column r0 are written
+//    back; the rest of its rank-2 update (R2 and R3 beyond column r0) then runs beside
+//    the next sweep's hop, which waits for prog[s-1] >= r1. A hop's reflector is thus
+//    formed in the hop before, so a hop runs: wait for sweep s-1, load the b new
+//    columns, p and kappa, R1 and column r0, the next reflector, write-back and
+//    progress, the rest of R2 and R3. The next sweep's first hop waits for columns up
+//    to s + 1 + b (published by hop 1), so hop 0 publishes nothing. The sweep-to-sweep
+//    lag is about one hop plus the part of the next hop before its publish; the
+//    reference waits for a margin of 3b columns (bulge.cu:193-199).
 //  * Hand-offs use agent-scope (write-through) band accesses and flags, with no L2
 //    write-back or invalidate (see ld_agent / st_agent below); every wait is bounded,
 //    and the grid never exceeds the number of co-resident workgroups, so the lowest
@@ -97,11 +100,11 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
     // k segments of the p product: (row, segment) pairs cover the workgroup's threads
     constexpr int NSEG = (kBCThreads / (3 * B)) < 1 ? 1 : (kBCThreads / (3 * B));
     T *win = reinterpret_cast<T *>(smem_raw);          // [NSLOT][WLD]
-    T *sw = win + (size_t)NSLOT * WLD;                 // [b]   reflector w
-    T *sp = sw + b;                                    // [3b]  p
+    T *sw = win + (size_t)NSLOT * WLD;                 // [2][b] reflectors: this hop's, the next hop's
+    T *sp = sw + 2 * b;                                // [3b]  p
     T *spart = sp + 3 * b;                             // [NSEG][3b] partial sums of p
     __shared__ T sred[NW];
-    __shared__ double s_beta;
+    __shared__ double s_beta[2];
     __shared__ int s_go;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -112,14 +115,28 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         const int hi_ = max(i, k), lo_ = min(i, k);
         return win[(lo_ & (NSLOT - 1)) * WLD + (hi_ - lo_)];
     };
-    // copy packed columns [c0, c1] between global memory and their ring slots
+    // copy packed columns [c0, c1] between global memory and their ring slots: in
+    // rounds of NEW_PER_THREAD loads per thread, all issued before any LDS store (one
+    // round trip per round; the sweep's first window of b + 1 columns takes two)
     auto load_cols = [&](int c0, int c1) {
         if (c1 < c0) return;
         const int cnt = (c1 - c0 + 1) * ldb;
         const T *src = Bp + (i64)c0 * ldb;
-        for (int idx = tid; idx < cnt; idx += kBCThreads) {
-            const int jj = idx / ldb, d = idx - jj * ldb;
-            win[((c0 + jj) & (NSLOT - 1)) * WLD + d] = ld_agent(src + idx);
+        for (int base = 0; base < cnt; base += NEW_PER_THREAD * kBCThreads) {
+            T v[NEW_PER_THREAD];
+#pragma unroll
+            for (int u = 0; u < NEW_PER_THREAD; ++u) {
+                const int idx = base + tid + u * kBCThreads;
+                v[u] = (idx < cnt) ? ld_agent(src + idx) : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < NEW_PER_THREAD; ++u) {
+                const int idx = base + tid + u * kBCThreads;
+                if (idx < cnt) {
+                    const int jj = idx / ldb, d = idx - jj * ldb;
+                    win[((c0 + jj) & (NSLOT - 1)) * WLD + d] = v[u];
+                }
+            }
         }
     };
     // exactly b new columns: all loads issued before any LDS store (one round trip)
@@ -162,8 +179,8 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
     };
 
     // Householder vector of x = A[r0_ : r0_ + L_ - 1, c_] (column c_ of the window),
-    // by wave 0 alone: w into sw, beta into s_beta
-    auto householder = [&](int r0_, int c_, int L_) {
+    // by wave 0 alone: w into wd, beta into *bd
+    auto householder = [&](int r0_, int c_, int L_, T *wd, double *bd) {
         const double x = (lane < L_) ? (double)W_(r0_ + lane, c_) : 0.0;
         const double alpha = __shfl(x, 0, 64);
         const double tail = wave_sum((lane >= 1) ? x * x : 0.0);
@@ -175,8 +192,18 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             const double inv = 1.0 / sqrt(v0 * v0 + tail);
             wv = ((lane == 0) ? v0 : x) * inv;
         }
-        if (lane < L_) sw[lane] = (T)wv;
-        if (lane == 0) s_beta = beta;
+        if (lane < L_) wd[lane] = (T)wv;
+        if (lane == 0) *bd = beta;
+    };
+    // column c_ is final: write it back, the eliminated entries [o_, o_ + L_) as beta e_0
+    // (skip_pivot: all but entry o_)
+    auto writeback = [&](int c_, int o_, int L_, double beta_, bool skip_pivot = false) {
+        T *dst = Bp + (i64)c_ * ldb;
+        const T *srcw = win + (size_t)(c_ & (NSLOT - 1)) * WLD;
+        for (int d = tid; d < ldb; d += kBCThreads) {
+            if (skip_pivot && d == o_) continue;
+            st_agent(dst + d, (d == o_) ? (T)beta_ : ((d > o_ && d < o_ + L_) ? T(0) : srcw[d]));
+        }
     };
 
     // this workgroup's sweeps in increasing order: the next one of the current phase,
@@ -192,7 +219,8 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
     int s = settle(((int)blockIdx.x < ph.g[0]) ? ph.s0[0] + (int)blockIdx.x : ph.s0[1]);
     for (; s < n - 2;) {
         int held_lo = 0, held_hi = -1; // window columns currently held in LDS
-        bool hh_ready = false;         // this hop's reflector was formed at the end of the last
+        T *wc = sw, *wn = sw + b;      // this hop's reflector, the next hop's
+        double *bc = &s_beta[0], *bn = &s_beta[1];
         for (int h = 0;; ++h) {
             const int cq = (h == 0) ? s : s + 1 + (h - 1) * b;
             const int r0 = (h == 0) ? s + 1 : cq + b;
@@ -236,35 +264,19 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                 __syncthreads();
                 return true;
             };
-            // After the first hop, column cq is already in LDS and final with respect
-            // to sweep s-1 (whose hops since the last wait touch columns > cq), so the
-            // reflector and the progress flag go first and the wait and the load
-            // overlap the next sweep's start.
-            if (h == 0 && !wait_and_load()) return;
-
-            // ---- Householder vector on x = A[r0:r1, cq] (wave 0), unless the previous
-            // hop formed it already (the barrier that ended that hop then covers it)
-            if (!hh_ready) {
-                if (wave == 0) householder(r0, cq, L);
+            if (!wait_and_load()) return;
+            if (h == 0) {
+                // the sweep's first reflector, on column s; column s is then final (rows
+                // above r0 untouched by this sweep, beta e_0 below): write it back and
+                // publish progress s. Later hops find their reflector formed and their
+                // column cq written back and published by the hop before.
+                // No progress is published for column s (or s + 1 below): the next
+                // sweep first waits for columns up to s + 1 + b, published by hop 1,
+                // whose fence also covers these stores.
+                if (wave == 0) householder(r0, cq, L, wc, bc);
                 __syncthreads();
+                writeback(cq, r0 - cq, L, *bc);
             }
-            hh_ready = false;
-
-            // ---- column cq is now final (rows above r0 untouched by this sweep, beta e_0
-            // below): write it back and publish progress cq (release) before the rest of
-            // the hop. The next sweep needs only this column from this hop (its window
-            // ends at column cq), so it overlaps the rest of the hop; this hop reads and
-            // writes column cq only in LDS from here on.
-            {
-                const int o = r0 - cq;
-                T *dst = Bp + (i64)cq * ldb;
-                const T *srcw = win + (size_t)(cq & (NSLOT - 1)) * WLD;
-                for (int d = tid; d < ldb; d += kBCThreads)
-                    st_agent(dst + d, (d == o) ? (T)s_beta : ((d > o && d < o + L) ? T(0) : srcw[d]));
-            }
-            wg_publish_fence();
-            if (tid == 0) flag_store_agent(prog + s, cq);
-            if (h > 0 && !wait_and_load()) return;
 
             // ---- p_t = 2 sum_k A(cq+t, r0+k) w_k, t < nrow. Thread = (row t, k segment);
             // each thread walks its segment starting at a row-dependent offset, so the
@@ -295,7 +307,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                         for (int j = 0; j < FL; ++j) {
                             const int q = kb + ((t + j) & (FL - 1));
                             const int k = r0 + q;
-                            a4[j & 3] += Wsym(i, k) * sw[q];
+                            a4[j & 3] += Wsym(i, k) * wc[q];
                         }
                         acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
                     } else {
@@ -304,7 +316,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                             const int q = kb + qq;
                             const int k = r0 + q;
                             const T a = Wsym(i, k);
-                            acc += a * sw[q];
+                            acc += a * wc[q];
                             qq = (qq + 1 == len) ? 0 : qq + 1;
                         }
                     }
@@ -317,7 +329,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                     if (seg == 0 && t < nrow) {
                         const T pv = T(2) * (acc + other);
                         sp[t] = pv;
-                        if (t >= o && t < o + L) kv = pv * sw[t - o];
+                        if (t >= o && t < o + L) kv = pv * wc[t - o];
                     }
                     kv = wave_sum(kv);
                     if (lane == 0) sred[wave] = kv;
@@ -335,7 +347,7 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
                     for (int v = 0; v < NSEG; ++v) ps += spart[v * (3 * B) + tid];
                     const T pv = T(2) * ps;
                     sp[tid] = pv;
-                    if (tid >= o && tid < o + L) kv = pv * sw[tid - o];
+                    if (tid >= o && tid < o + L) kv = pv * wc[tid - o];
                 }
                 kv = wave_sum(kv);
                 if (lane == 0) sred[wave] = kv;
@@ -348,84 +360,136 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
             //       eliminated column, set to its exact value beta e_0
             //   R2: rows r0+a (a < L)      x columns r0+c, c <= a     (both non-zero)
             //   R3: rows r1+1+a (a < m3)   x columns r0+c, c < L      (w_i = 0)
-            // thread = (row a, column group cg); the column value is a broadcast read
-            {
+            // thread = (row a, column group cg); the column value is a broadcast read.
+            // part 1: R1, and column c = 0 of R2 and R3 (column r0, the next hop's
+            // reflector column); part 2: the rest of R2 and R3; part 3: all of it
+            auto rank2 = [&](int part) {
+                const bool dR1 = part != 2, dC0 = part != 2, dRest = part != 1;
                 T kap = T(0);
 #pragma unroll
                 for (int v = 0; v < NW; ++v) kap += sred[v];
                 constexpr int NG = kBCThreads / B;
                 const int a = tid % B, cg = tid / B;
                 const int m3 = hi - r1;
-                const T wa = (a < L) ? sw[a] : T(0);
+                const T wa = (a < L) ? wc[a] : T(0);
                 const T ya = (a < L) ? sp[o + a] - kap * wa : T(0);
                 const T y3 = (a < m3) ? sp[o + L + a] : T(0);
+                const T bet = (T)*bc;
                 constexpr bool kFull = B >= NG && B % NG == 0; // whole columns per thread group
-                if (kFull && o == B && L == B && m3 == B) {
+                // R1 has o < B columns in a sweep's first hop (o = 1): it then goes through
+                // the general loop, and R2 and R3 through the fast path below
+                const bool full23 = kFull && L == B && m3 == B;
+                const bool split1 = full23 && o != B;
+                if (split1 && dR1 && a < L)
+                    for (int cc = cg; cc < o; cc += NG) {
+                        T &e = W_(r0 + a, cq + cc);
+                        if (cc == 0) e = (a == 0) ? bet : T(0);
+                        else e -= wa * sp[cc];
+                    }
+                if (full23) {
+                    const bool fR1 = dR1 && !split1;
                     // full hop (every thread in all three regions, each element read and
                     // written by its own thread only): every LDS read before any write,
                     // so the round trips overlap; same arithmetic as the general path
                     constexpr int NU = kFull ? B / NG : 1;
-                    T e1[NU], e2[NU], e3[NU], p1[NU], wc[NU], pc[NU];
+                    T e1[NU], e2[NU], e3[NU], p1[NU], wcv[NU], pc[NU];
 #pragma unroll
                     for (int u = 0; u < NU; ++u) {
                         const int cc = cg + NG * u;
-                        e1[u] = W_(r0 + a, cq + cc);
-                        p1[u] = sp[cc];
-                        e2[u] = W_(r0 + a, r0 + min(cc, a));
-                        wc[u] = sw[cc];
-                        pc[u] = sp[o + cc];
-                        e3[u] = W_(r1 + 1 + a, r0 + cc);
+                        const bool d23 = (cc == 0) ? dC0 : dRest;
+                        if (fR1) {
+                            e1[u] = W_(r0 + a, cq + cc);
+                            p1[u] = sp[cc];
+                        }
+                        if (d23) {
+                            e2[u] = W_(r0 + a, r0 + min(cc, a));
+                            wcv[u] = wc[cc];
+                            pc[u] = sp[o + cc];
+                            e3[u] = W_(r1 + 1 + a, r0 + cc);
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < NU; ++u) {
                         const int cc = cg + NG * u;
-                        W_(r0 + a, cq + cc) = (cc == 0) ? ((a == 0) ? (T)s_beta : T(0)) : e1[u] - wa * p1[u];
-                        if (cc <= a) W_(r0 + a, r0 + cc) = e2[u] - (wa * (pc[u] - kap * wc[u]) + ya * wc[u]);
-                        W_(r1 + 1 + a, r0 + cc) = e3[u] - y3 * wc[u];
+                        const bool d23 = (cc == 0) ? dC0 : dRest;
+                        if (fR1) W_(r0 + a, cq + cc) = (cc == 0) ? ((a == 0) ? bet : T(0)) : e1[u] - wa * p1[u];
+                        if (d23) {
+                            if (cc <= a) W_(r0 + a, r0 + cc) = e2[u] - (wa * (pc[u] - kap * wcv[u]) + ya * wcv[u]);
+                            W_(r1 + 1 + a, r0 + cc) = e3[u] - y3 * wcv[u];
+                        }
                     }
                 } else {
                     if (a < L) {
-                        for (int cc = cg; cc < o; cc += NG) {
-                            T &e = W_(r0 + a, cq + cc);
-                            if (cc == 0) e = (a == 0) ? (T)s_beta : T(0);
-                            else e -= wa * sp[cc];
-                        }
+                        if (dR1)
+                            for (int cc = cg; cc < o; cc += NG) {
+                                T &e = W_(r0 + a, cq + cc);
+                                if (cc == 0) e = (a == 0) ? bet : T(0);
+                                else e -= wa * sp[cc];
+                            }
                         for (int cc = cg; cc <= a; cc += NG) {
-                            const T wc = sw[cc];
-                            const T yc = sp[o + cc] - kap * wc;
-                            W_(r0 + a, r0 + cc) -= wa * yc + ya * wc;
+                            if (!((cc == 0) ? dC0 : dRest)) continue;
+                            const T wcc = wc[cc];
+                            const T yc = sp[o + cc] - kap * wcc;
+                            W_(r0 + a, r0 + cc) -= wa * yc + ya * wcc;
                         }
                     }
                     if (a < m3) {
-                        for (int cc = cg; cc < L; cc += NG) W_(r1 + 1 + a, r0 + cc) -= y3 * sw[cc];
+                        for (int cc = cg; cc < L; cc += NG) {
+                            if (!((cc == 0) ? dC0 : dRest)) continue;
+                            W_(r1 + 1 + a, r0 + cc) -= y3 * wc[cc];
+                        }
                     }
                 }
-            }
-            __syncthreads();
+            };
 
-            // ---- columns this sweep never touches again go back to global memory:
-            // [cq, next cq - 1] (the next hop starts at s+1 after hop 0, at cq+b after)
-            // (column cq itself went out with the progress flag; the next hop's flag
-            // releases these stores)
-            const int next_cq = (h == 0) ? s + 1 : cq + b;
-            const int keep_lo = min(next_cq, r1 + 1);
             T *ucol = U + (i64)(r0 - s - 1) + (i64)s * ldu;
-            for (int t = tid; t < L; t += kBCThreads) ucol[t] = sw[t];
-            // the next hop's reflector: its column next_cq is final after the rank-2
-            // update (and after sweep s-1, which waited for columns <= r1), it is not
-            // among the columns stored below, and sw / s_beta are no longer read by
-            // this hop except for U just above (wave 0's own lanes); wave 0 forms it
-            // while the other waves start the stores
-            {
-                const int nr0 = next_cq + b, nr1 = min(nr0 + b - 1, n - 1);
-                hh_ready = nr0 <= n - 1 && nr1 - nr0 + 1 >= 2;
-                if (hh_ready && wave == 0) householder(nr0, next_cq, nr1 - nr0 + 1);
+            // the next hop: cq' = r0, its reflector on rows [r0 + b, ...] of column r0
+            const int nr0 = r0 + b, nr1 = min(nr0 + b - 1, n - 1);
+            const bool next_hh = nr0 <= n - 1 && nr1 - nr0 + 1 >= 2;
+            if (next_hh) {
+                // Early publish. The next sweep's hop that waits for this one reads the
+                // columns up to r0 only: R1's columns [cq, r0) and column r0, which are
+                // final once R1 and column r0 are updated and the next hop's reflector has
+                // eliminated column r0 (this sweep later writes column r0 only with those
+                // same values, and reads it only in LDS). So: R1 and column r0; the next
+                // reflector (wave 0) while the other waves store R1's columns and U; column
+                // r0 written back; progress r0 published; then the rest of R2 and R3,
+                // beside the next sweep's hop.
+                rank2(1);
+                __syncthreads();
+                if (wave == 0) householder(nr0, r0, nr1 - nr0 + 1, wn, bn);
+                for (int t = tid; t < L; t += kBCThreads) ucol[t] = wc[t];
+                store_cols(cq + 1, r0 - 1);
+                // column r0 except its pivot entry (b, the only value the reflector
+                // decides), then the pivot once the reflector is formed
+                writeback(r0, b, nr1 - nr0 + 1, T(0), true);
+                __syncthreads();
+                if (tid == 0) st_agent(Bp + (i64)r0 * ldb + b, (T)*bn);
+                if (h > 0) {
+                    wg_publish_fence();
+                    if (tid == 0) flag_store_agent(prog + s, r0);
+                }
+                rank2(2);
+                held_lo = r0;
+                // the next hop's loads overwrite the ring slots of the columns stored above,
+                // and its p reads R2 and R3
+                __syncthreads();
+                T *tw = wc;
+                wc = wn;
+                wn = tw;
+                double *tb = bc;
+                bc = bn;
+                bn = tb;
+            } else {
+                // the sweep's last reflector
+                rank2(3);
+                __syncthreads();
+                for (int t = tid; t < L; t += kBCThreads) ucol[t] = wc[t];
+                const int keep_lo = min(r0, r1 + 1);
+                store_cols(cq + 1, keep_lo - 1);
+                held_lo = keep_lo;
+                __syncthreads();
             }
-            store_cols(cq + 1, keep_lo - 1);
-            held_lo = keep_lo;
-            // the next hop's loads overwrite the ring slots just stored (sweep 0 has no
-            // wait phase, whose barrier would otherwise separate them)
-            __syncthreads();
         }
         // the rest of the last window
         store_cols(held_lo, held_hi);
@@ -446,7 +510,7 @@ __global__ void bc_extract_kernel(const T *Bp, T *d, T *e, int n, int b) {
 
 template <typename T> size_t bc_lds_bytes(int b) {
     const size_t nseg = std::max<size_t>(1, kBCThreads / (3 * (size_t)b));
-    return sizeof(T) * ((size_t)2 * b * (2 * b + 1) + b + 3 * b + nseg * 3 * b);
+    return sizeof(T) * ((size_t)2 * b * (2 * b + 1) + 2 * b + 3 * b + nseg * 3 * b);
 }
 
 } // namespace
