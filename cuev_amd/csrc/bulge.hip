@@ -116,21 +116,22 @@ __global__ __launch_bounds__(kBCThreads) void bc_chase_kernel(T *Bp, T *U, i64 l
         return win[(lo_ & (NSLOT - 1)) * WLD + (hi_ - lo_)];
     };
     // copy packed columns [c0, c1] between global memory and their ring slots: in
-    // rounds of NEW_PER_THREAD loads per thread, all issued before any LDS store (one
-    // round trip per round; the sweep's first window of b + 1 columns takes two)
+    // rounds of FIRST_PER_THREAD loads per thread, all issued before any LDS store (one
+    // round trip per round; a sweep's first window of b + 1 columns takes one)
+    constexpr int FIRST_PER_THREAD = (2 * B * (B + 1) + kBCThreads - 1) / kBCThreads;
     auto load_cols = [&](int c0, int c1) {
         if (c1 < c0) return;
         const int cnt = (c1 - c0 + 1) * ldb;
         const T *src = Bp + (i64)c0 * ldb;
-        for (int base = 0; base < cnt; base += NEW_PER_THREAD * kBCThreads) {
-            T v[NEW_PER_THREAD];
+        for (int base = 0; base < cnt; base += FIRST_PER_THREAD * kBCThreads) {
+            T v[FIRST_PER_THREAD];
 #pragma unroll
-            for (int u = 0; u < NEW_PER_THREAD; ++u) {
+            for (int u = 0; u < FIRST_PER_THREAD; ++u) {
                 const int idx = base + tid + u * kBCThreads;
                 v[u] = (idx < cnt) ? ld_agent(src + idx) : T(0);
             }
 #pragma unroll
-            for (int u = 0; u < NEW_PER_THREAD; ++u) {
+            for (int u = 0; u < FIRST_PER_THREAD; ++u) {
                 const int idx = base + tid + u * kBCThreads;
                 if (idx < cnt) {
                     const int jj = idx / ldb, d = idx - jj * ldb;
