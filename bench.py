@@ -21,7 +21,9 @@ copied to the host). With --mixed the FP32 + FP64 refinement form is timed on th
 same matrix as well.
 
 The default q16k run adds the metric's other two numbers as extra keys (skip them
-with --no-extras): "g49k", one timed n = 49152 solve after one untimed one, with its
+with --no-extras), and "values_only", the eigenvalues-only path (band reduction, chase
+and Sturm multisection) timed on the same matrix: "g49k", one timed n = 49152 solve
+after one untimed one, with its
 accuracy checked over column blocks of V; and "mixed", the mixed16k matrix (n = 16384,
 geometric clustered spectrum) solved by the FP64 path and by the mixed FP32 +
 refinement path, with both times, the speedup and the accuracy of each.
@@ -331,6 +333,26 @@ def main():
     flops16_3 = 16.0 / 3.0 * n ** 3
     dbbr_tf = (4.0 / 3.0 * n ** 3) / (stages["dbbr_ms"] * 1e-3) / 1e12 if stages["dbbr_ms"] else None
 
+    # eigenvalues only (band reduction, chase, Sturm multisection; no back-transform, no
+    # D&C) on the same matrix: one untimed and one timed call
+    values = None
+    if not a.no_extras:
+        lam_v = None
+        for i in range(2):
+            work.mT.copy_(A0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            lam_v, tmv = cv.symm_eig_values(work.mT, overwrite=True, bandwidth=b, block=k, timer=True)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+        a2 = lam_v.double().abs().max().item()
+        values = {"seconds": round(dt, 4), "speedup_vs_eigenpairs": round(mean_s / dt, 3),
+                  "multisection_ms": round(tmv["dc_ms"], 2)}
+        if lam_exact is not None:
+            values["eig_err_vs_exact_over_norm2"] = (lam_v.double() - lam_exact).abs().max().item() / a2
+        log(f"[bench] eigenvalues only: {dt:.3f} s {values}")
+        del lam_v
+
     mixed = None
     if a.mixed:
         try:
@@ -391,6 +413,7 @@ def main():
                     for t, e in kernels.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "values_only": values,
         "g49k": g49k,
         "mixed": mixed,
         "lapack": cv.lapack_name(),

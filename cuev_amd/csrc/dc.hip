@@ -51,6 +51,8 @@ struct MergeDesc {
     i64 chunk;      // element offset of this merge's Qc / S / Qn blocks (sum of N^2 before it)
     int off, N, n1; // rows and columns [off, off + N); the cut after n1
     int k, k1, k2;  // non-deflated count; of them k1 top-only, k2 dense, k - k1 - k2 bottom-only
+    int ks;         // leading dimension of S (k, or k + 1 to make it even)
+    int zb;         // 1: Qc's bottom block starts with a zero column (see the merge GEMMs)
     int rot_begin, rot_end; // Givens rotations of the deflation
     int root_begin;         // flat index of the first non-deflated pole
     int defl_begin;         // flat index of the first deflated entry
@@ -59,7 +61,7 @@ struct MergeDesc {
 
 struct CopyJob {
     i64 src, dst;
-    int len, pad;
+    int len, zero; // zero != 0: write len zeros at dst (src unused)
 };
 
 struct LeafDesc {
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256) void dc_rot_kernel(T *Q, i64 ldq, const MergeD
 template <typename T>
 __global__ __launch_bounds__(256) void dc_copy_kernel(const T *src, T *dst, const CopyJob *jobs) {
     const CopyJob j = jobs[blockIdx.x];
-    for (int r = threadIdx.x; r < j.len; r += 256) dst[j.dst + r] = src[j.src + r];
+    for (int r = threadIdx.x; r < j.len; r += 256) dst[j.dst + r] = j.zero ? T(0) : src[j.src + r];
 }
 
 // Secular roots, one wave per root. Root j (0-based) of merge m lies in
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(256) void dc_s_kernel(const MergeDesc *md, const in
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
     __syncthreads();
     const T inv = T(1) / sqrt(red[0] + red[1] + red[2] + red[3]);
-    T *col = S + m.chunk + (i64)j * k;
+    T *col = S + m.chunk + (i64)j * m.ks;
     for (int p = threadIdx.x; p < k; p += 256) {
         const int i = P[p];
         col[p] = (Z[i] / ((D[i] - dor) - t)) * inv;
@@ -747,12 +749,21 @@ int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q,
                 t2i[nroot + p] = h.t2i[p];
                 rmerge[nroot + p] = i;
             }
-            const int ktop = h.k1 + h.k2;
+            const int ktop = h.k1 + h.k2, kbot = k - h.k1;
+            // The merge GEMMs read S with an even leading dimension and the bottom block's
+            // S rows from an even row, so that both operands can take the engine's 16-byte
+            // (LDS-DMA) path: S gets ld k + 1 when k is odd, and when k1 is odd the bottom
+            // block of Qc starts with a zero column and its product reads S from row k1 - 1
+            // (a top-only row, which the zero column cancels). Each only when it fits the
+            // merge's N^2 block.
+            m.ks = ((k & 1) && (i64)(k + 1) * k <= (i64)N * N) ? k + 1 : k;
+            m.zb = ((h.k1 & 1) && (i64)n1 * ktop + (i64)n2 * (kbot + 1) <= (i64)N * N) ? 1 : 0;
             for (int p = 0; p < ktop; ++p)
                 gjob[ng++] = CopyJob{(i64)m.off + (i64)h.gcol[p] * ldq, chunk + (i64)p * n1, n1, 0};
+            if (m.zb) gjob[ng++] = CopyJob{0, chunk + (i64)n1 * ktop, n2, 1};
             for (int p = h.k1; p < k; ++p)
                 gjob[ng++] = CopyJob{(i64)(m.off + n1) + (i64)h.gcol[p] * ldq,
-                                     chunk + (i64)n1 * ktop + (i64)(p - h.k1) * n2, n2, 0};
+                                     chunk + (i64)n1 * ktop + (i64)(p - h.k1 + m.zb) * n2, n2, 0};
             m.defl_begin = ndefl;
             for (size_t t = 0; t < h.defl.size(); ++t) {
                 dv[ndefl + (int)t] = h.defl[t].first;
@@ -824,9 +835,9 @@ int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q,
                 const T *qc = Qc + m.chunk;
                 const T *sm = S + m.chunk;
                 T *qn = Qn + m.chunk;
-                CUEV_TRY(gemm<T>(s, 0, 0, n1, k, ktop, T(1), qc, n1, sm, k, T(0), qn, N, gs));
-                CUEV_TRY(gemm<T>(s, 0, 0, n2, k, kbot, T(1), qc + (i64)n1 * ktop, std::max(n2, 1),
-                                 sm + m.k1, k, T(0), qn + n1, N, gs));
+                CUEV_TRY(gemm<T>(s, 0, 0, n1, k, ktop, T(1), qc, n1, sm, m.ks, T(0), qn, N, gs));
+                CUEV_TRY(gemm<T>(s, 0, 0, n2, k, kbot + m.zb, T(1), qc + (i64)n1 * ktop, std::max(n2, 1),
+                                 sm + m.k1 - m.zb, m.ks, T(0), qn + n1, N, gs));
             }
         }
         {
