@@ -57,7 +57,8 @@ def test_gemm_beta_zero_ignores_nan(torch_dev):
     assert np.allclose(from_dev(dC), 30.0)
 
 
-@pytest.mark.parametrize("m,n", [(5, 3), (200, 64), (1000, 64), (3000, 64), (513, 200)])
+@pytest.mark.parametrize("m,n", [(5, 3), (200, 64), (1000, 64), (3000, 64), (513, 200), (4096, 64),
+                                 (8190, 64), (2050, 48)])
 def test_symm_lower(torch_dev, m, n):
     torch = torch_dev
     rng = np.random.default_rng(m + n)
