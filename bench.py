@@ -40,6 +40,9 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix peak, FLOP/s
 PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12          # bytes/s
+# single-launch sites whose HIP-event durations the timed steps record (the roofline's
+# candidates for the dominant kernel)
+ROOFLINE_SITES = ("bc.chase", "bt.bc_back", "bt.gemm")
 
 
 def log(*a):
@@ -285,7 +288,11 @@ def main():
         dt, _, _, _ = one()
         log(f"[bench] warmup {i}: {dt:.3f} s")
 
-    cv.profile_enable(True)
+    # the timed steps record events only around the single-launch sites the roofline
+    # reads: every recorded site puts two event records on the stream, which hold
+    # back-to-back kernels apart by about 10 us each (about 1% of a solve if every
+    # site were recorded)
+    cv.profile_enable(True, only=ROOFLINE_SITES)
     cv.profile_read(reset=True)
     times, timers = [], []
     for i in range(a.steps):
@@ -295,9 +302,14 @@ def main():
         log(f"[bench] step {i}: {dt:.3f} s  dbbr={tm['dbbr_ms']:.1f} bc={tm['bc_ms']:.1f} "
             f"dc={tm['dc_ms']:.1f} bt_after_dc={tm['bt_ms']:.1f} (overlapped {tm['overlap_ms']:.1f}) ms")
     prof = cv.profile_read(reset=True)
-    cv.profile_enable(False)
     K = len(times)
     mean_s = sum(times) / K
+    # every launch site, over one more (untimed) solve, for the per-kernel table
+    cv.profile_enable(True)
+    cv.profile_read(reset=True)
+    one()
+    kprof = cv.profile_read(reset=True)
+    cv.profile_enable(False)
 
     # ---- correctness of the last solve
     check = {}
@@ -314,9 +326,9 @@ def main():
 
     # ---- per-kernel rates (library HIP events over the timed steps)
     kernels = {}
-    for tag, r in prof.items():
+    for tag, r in kprof.items():
         per_launch_ms = r["ms"] / max(1, r["count"])
-        ent = {"ms_per_step": r["ms"] / K, "launches_per_step": r["count"] / K,
+        ent = {"ms_per_step": r["ms"], "launches_per_step": r["count"],
                "avg_launch_ms": per_launch_ms}
         if r["flops"] > 0 and r["ms"] > 0:
             ent["tflops"] = r["flops"] / (r["ms"] * 1e-3) / 1e12
@@ -411,6 +423,9 @@ def main():
         "check": check,
         "kernels": {t: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in e.items()}
                     for t, e in kernels.items()},
+        "kernels_source": "library HIP events around every launch site over one untimed solve "
+                          "after the timed steps (the timed steps record only " +
+                          ", ".join(ROOFLINE_SITES) + ")",
         "roofline": roofline,
         "cpu_baseline": cpu,
         "values_only": values,
@@ -429,8 +444,7 @@ def dominant_roofline(prof, n, b):
     n^2 / (2b) hops), over its HIP-event duration; "frac_model_5p5bn2S" gives the same
     duration against the 5.5 b n^2 S L2-traffic model of a kernel that re-reads its
     window every hop. MFMA-bound kernels: the flops the library records per launch."""
-    single = {"bc.chase": "hbm", "bt.bc_back": "mfma", "bt.gemm": "mfma", "dbbr.syr2k": "mfma",
-              "dbbr.symm": "mfma"}
+    single = {"bc.chase": "hbm", "bt.bc_back": "mfma", "bt.gemm": "mfma"}
     dom = max((t for t in single if t in prof), key=lambda t: prof[t]["ms"], default=None)
     if not dom:
         return None
@@ -459,11 +473,13 @@ def time_mixed(torch, cv, one, A0, lam_exact, warmup, steps, fp64_s):
     for i in range(warmup):
         one("mixed")
     mt = []
-    cv.profile_enable(True)
-    cv.profile_read(reset=True)
     for i in range(steps):
         dt, lam_m, V_m, tm_m = one("mixed")
         mt.append(dt)
+    # per-site event times over one more (untimed) mixed solve
+    cv.profile_enable(True)
+    cv.profile_read(reset=True)
+    one("mixed")
     mprof = cv.profile_read(reset=True)
     cv.profile_enable(False)
     res_m, orth_m = residual_orth(torch, A0, V_m, lam_m.double())
@@ -476,7 +492,7 @@ def time_mixed(torch, cv, one, A0, lam_exact, warmup, steps, fp64_s):
              "stages_ms": {kk: round(float(v), 2) for kk, v in tm_m.items() if kk.endswith("_ms")},
              # library events per mixed solve (the Rayleigh-Ritz step's inner FP64
              # solve shows under the FP64 stage names)
-             "kernels_ms": {t: round(r["ms"] / len(mt), 2) for t, r in mprof.items()}}
+             "kernels_ms": {t: round(r["ms"], 2) for t, r in mprof.items()}}
     if lam_exact is not None:
         mixed["eig_err_vs_exact_over_norm2"] = \
             (lam_m.double() - lam_exact).abs().max().item() / lam_exact.abs().max().item()

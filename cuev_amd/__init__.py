@@ -111,6 +111,7 @@ def _declare(L):
         getattr(L, f"cuev_{pre}syr2k_lower").argtypes = [P, I64, I64, sc, P, I64, P, I64, sc, P, I64]
     L.cuev_cache_release.argtypes = []
     L.cuev_profile_enable.argtypes = [I]
+    L.cuev_profile_select.argtypes = [ctypes.c_char_p]
     L.cuev_profile_read.argtypes = [ctypes.c_char_p, I, I]
     L.cuev_handle_create.argtypes = [ctypes.POINTER(P), I, I, I, I, P]
     L.cuev_handle_destroy.argtypes = [P]
@@ -124,8 +125,16 @@ def _check(st):
         raise CuevError(st, msg)
 
 
-def profile_enable(on=True):
-    """Record per-launch HIP event timings inside the library (see profile_read)."""
+def profile_enable(on=True, only=None):
+    """Record per-launch HIP event timings inside the library (see profile_read).
+
+    only: an iterable of tag prefixes (or one comma-separated string) to record;
+    None records every tag. Every recorded launch puts two event records on its
+    stream, a few microseconds each between back-to-back kernels, so a timed run
+    should record only the tags it reports."""
+    if only is not None and not isinstance(only, str):
+        only = ",".join(only)
+    lib().cuev_profile_select(only.encode() if only else None)
     lib().cuev_profile_enable(1 if on else 0)
 
 

@@ -623,6 +623,43 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int s
     *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
 }
 
+// The same sum for few outputs and many splits (the band reduction's w x b and b x b
+// products over the whole panel height): each 64 outputs get 16 waves, wave g summing
+// the splits g, g + 16, ... (lanes over consecutive outputs, so every load is one
+// coalesced row), and the 16 wave sums are added in a fixed order.
+constexpr int kWideWaves = 16;
+template <typename T>
+__global__ __launch_bounds__(64 * kWideWaves) void splitk_reduce_wide_kernel(const T *part, int splits, i64 M,
+                                                                             i64 N, T alpha, T beta, T *C,
+                                                                             i64 ldc, int lower) {
+    __shared__ T red[kWideWaves][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const i64 MN = M * N;
+    const i64 idx = (i64)blockIdx.x * 64 + lane;
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    if (idx < MN) {
+        const T *q = part + idx;
+        int k = g;
+        for (; k + 3 * kWideWaves < splits; k += 4 * kWideWaves) {
+            s0 += q[(i64)k * MN];
+            s1 += q[(i64)(k + kWideWaves) * MN];
+            s2 += q[(i64)(k + 2 * kWideWaves) * MN];
+            s3 += q[(i64)(k + 3 * kWideWaves) * MN];
+        }
+        for (; k < splits; k += kWideWaves) s0 += q[(i64)k * MN];
+    }
+    red[g][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (g != 0 || idx >= MN) return;
+    const i64 m = idx % M, n = idx / M;
+    if (lower && m < n) return;
+    T s = T(0);
+#pragma unroll
+    for (int w = 0; w < kWideWaves; ++w) s += red[w][lane];
+    T *c = C + m + n * ldc;
+    *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
+}
+
 // Tile configurations: (BM, BN, WM, WN, blocks per CU the launch bounds ask for).
 enum Cfg { C128x128 = 0, C128x64 = 1, C64x128 = 2, C64x64 = 3, C256x128 = 4 };
 
@@ -796,9 +833,17 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     if (p.part != nullptr || splits == 0) {
         const i64 total = M * N;
         const int nsplit = (splits == 0) ? 0 : splits;
-        hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)ceil_div(total, 256)),
-                           dim3(256), 0, s, static_cast<const T *>(p.part), nsplit, M, N, alpha,
-                           beta, C, ldc, lower);
+        if (nsplit >= 2 * kWideWaves && ceil_div(total, 256) < 2 * cus) {
+            // fewer than two blocks per CU of the plain sum, each thread summing many
+            // splits: give every output 16 waves instead
+            hipLaunchKernelGGL(splitk_reduce_wide_kernel<T>, dim3((unsigned)ceil_div(total, 64)),
+                               dim3(64 * kWideWaves), 0, s, static_cast<const T *>(p.part), nsplit, M, N,
+                               alpha, beta, C, ldc, lower);
+        } else {
+            hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)ceil_div(total, 256)),
+                               dim3(256), 0, s, static_cast<const T *>(p.part), nsplit, M, N, alpha,
+                               beta, C, ldc, lower);
+        }
         CUEV_LAUNCH_CHECK();
     }
     return CUEV_STATUS_SUCCESS;

@@ -510,9 +510,24 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
     if (g == 0 && tid == 0) __hip_atomic_store(a.reset, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int r0g = g * CH;
     const int chl = max(0, min(CH, a.rows - r0g));
-    for (int idx = tid; idx < B * CH; idx += kCQThreads) {
-        const int j = idx / CH, rl = idx - j * CH;
-        P[(size_t)j * CHp + rl] = (rl < chl) ? (double)a.A[(i64)(r0g + rl) + (i64)j * a.lda] : 0.0;
+    {
+        // the panel slice: rounds of LB loads per thread, all issued before their LDS stores
+        constexpr int LB = 8;
+        for (int base = 0; base < B * CH; base += LB * kCQThreads) {
+            double v[LB];
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const int idx = base + q * kCQThreads + tid;
+                const int j = idx / CH, rl = idx - j * CH;
+                v[q] = (idx < B * CH && rl < chl) ? (double)a.A[(i64)(r0g + rl) + (i64)j * a.lda] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const int idx = base + q * kCQThreads + tid;
+                const int j = idx / CH, rl = idx - j * CH;
+                if (idx < B * CH) P[(size_t)j * CHp + rl] = v[q];
+            }
+        }
     }
     __syncthreads();
 
@@ -577,24 +592,36 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
                 __syncthreads();
             }
-            const bool small = red[0] <= 1e-6;
+            // |E| <= 1e-10 entrywise (the usual case): up(E)^T up(E) and F^2 have entries
+            // below B 1e-20, under half an ulp of R_2's unit diagonal, so F = up(E) and
+            // R_2^-1 = I - F, and only F R_1 is formed
+            const bool small = red[0] <= 1e-6, tiny = red[0] <= 1e-10;
             __syncthreads();
             if (small) {
-                double c[4][4];
-                cq_mm<B, true>(c, M2, M2, si, sj); // up(E)^T up(E)
-                __syncthreads();
-                if (own)
+                if (!tiny) {
+                    double c[4][4];
+                    cq_mm<B, true>(c, M2, M2, si, sj); // up(E)^T up(E)
+                    __syncthreads();
+                    if (own)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int i = si + NB * u, j = sj + NB * v;
+                                const double f = e[u][v] - c[u][v];
+                                M2[i + j * LD] = (i < j) ? f : (i == j ? 0.5 * f : 0.0);
+                            }
+                    __syncthreads();
+                }
+                double f2[4][4], fr[4][4];
+                if (!tiny) {
+                    cq_mm<B, false>(f2, M2, M2, si, sj); // F^2
+                } else {
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const int i = si + NB * u, j = sj + NB * v;
-                            const double f = e[u][v] - c[u][v];
-                            M2[i + j * LD] = (i < j) ? f : (i == j ? 0.5 * f : 0.0);
-                        }
-                __syncthreads();
-                double f2[4][4], fr[4][4];
-                cq_mm<B, false>(f2, M2, M2, si, sj); // F^2
+                        for (int v = 0; v < 4; ++v) f2[u][v] = 0.0;
+                }
                 cq_mm<B, false>(fr, M2, M1, si, sj); // F R_1
                 double fo[4][4], r1[4][4];
                 if (own)
@@ -834,56 +861,73 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
 #pragma unroll
                     for (int v = 0; v < 4; ++v) M1[(si + NB * u) + (sj + NB * v) * LD] = acc[u][v];
         }
-        // U^-1 by back substitution, one row per step: rhs rows i < k -= U(i, k) V(k, :)
-        double vr[4][4];
+        // U^-1 by recursive doubling, into P's top B rows (V(i, j) at V[j * CHp + i]; L^-1
+        // is no longer needed there once T is formed): each of the B/4 diagonal 4 x 4 blocks
+        // by one thread, then merges of two inverted diagonal blocks of size h into one of
+        // size 2h, V12 = -V11 (U12 V22). X = U12 V22 is staged in xs (the 8B doubles from
+        // cb to gd, free here) in chunks of at most 16 columns. Only the upper triangles
+        // of the diagonal blocks are read, so V's lower part is never formed.
+        __syncthreads(); // the T product is done with P's top rows
+        double *V = P;
+        double *xs = cb;
+        if (tid < B / 4) {
+            const int c0 = 4 * tid;
+            double u[4][4], v[4][4], rd[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+            for (int i = 0; i < 4; ++i) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) vr[u][v] = (4 * ti + u == 4 * tj + v) ? 1.0 : 0.0;
-        if (own && ti == NB - 1)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                vr[3][v] /= Um[(B - 1) + (B - 1) * LD];
-                xb[((B - 1) & 1) * B + 4 * tj + v] = vr[3][v];
+                for (int j = 0; j < 4; ++j) u[i][j] = Um[(c0 + i) + (c0 + j) * LD];
+                rd[i] = 1.0 / u[i][i];
             }
-        for (int kb = NB - 1; kb >= 0; --kb) {
 #pragma unroll
-            for (int v0 = 3; v0 >= 0; --v0) {
-                const int k = 4 * kb + v0, p = k & 1;
-                if (k == 0) break;
-                __syncthreads();
-                double ukm[4], vk[4];
+            for (int j = 0; j < 4; ++j) {
+                v[j][j] = rd[j];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    ukm[u] = (4 * ti + u < k) ? Um[(4 * ti + u) + k * LD] : 0.0;
-                    vk[u] = xb[p * B + 4 * tj + u];
+                for (int i = j - 1; i >= 0; --i) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = i + 1; k <= j; ++k) t += u[i][k] * v[k][j];
+                    v[i][j] = -t * rd[i];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) vr[u][v] -= ukm[u] * vk[v];
-                const int kp = k - 1, up = (v0 + 3) & 3, bp = kp >> 2;
-                if (own && ti == bp) {
-                    const double d = 1.0 / Um[kp + kp * LD];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        vr[up][v] *= d;
-                        xb[(kp & 1) * B + 4 * tj + v] = vr[up][v];
-                    }
+                for (int i = 0; i <= j; ++i) V[(size_t)(c0 + j) * CHp + c0 + i] = v[i][j];
+            }
+        }
+        for (int h = 4; h < B; h *= 2) {
+            const int cw = h < 16 ? h : 16;
+            constexpr int HR = B / 2; // rows of X over all merges of a level
+            for (int j0 = 0; j0 < h; j0 += cw) {
+                __syncthreads(); // V22 (and the previous chunk's reads of xs) complete
+                for (int e = tid; e < HR * cw; e += kCQThreads) {
+                    const int rr = e % HR, jj = e / HR, q = rr / h, i = rr % h;
+                    const int c = 2 * h * q, j = j0 + jj;
+                    // four partial sums, so the LDS reads of consecutive k overlap
+                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+                    for (int k = 0; k <= j; ++k)
+                        t4[k & 3] += Um[(c + i) + (c + h + k) * LD] * V[(size_t)(c + h + j) * CHp + c + h + k];
+                    xs[rr + jj * HR] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                }
+                __syncthreads();
+                for (int e = tid; e < HR * cw; e += kCQThreads) {
+                    const int rr = e % HR, jj = e / HR, q = rr / h, i = rr % h;
+                    const int c = 2 * h * q, j = j0 + jj;
+                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+                    for (int k = i; k < h; ++k) t4[k & 3] += V[(size_t)(c + k) * CHp + c + i] * xs[(q * h + k) + jj * HR];
+                    V[(size_t)(c + h + j) * CHp + c + i] = -((t4[0] + t4[1]) + (t4[2] + t4[3]));
                 }
             }
         }
         __syncthreads();
-        if (own)
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int i = 4 * ti + u, j = 4 * tj + v;
-                    M3[i + j * LD] = vr[u][v];
-                    st_wt(a.bc + i + (size_t)j * B, vr[u][v]);
-                }
+        for (int idx = tid; idx < BB; idx += kCQThreads) {
+            const int i = idx % B, j = idx / B;
+            const double v = (i <= j) ? V[(size_t)j * CHp + i] : 0.0;
+            M3[i + j * LD] = v;
+            st_wt(a.bc + idx, v);
+        }
         for (int idx = tid; idx < BB; idx += kCQThreads) st_wt(a.bc + BB + idx, M1[(idx % B) + (idx / B) * LD]);
+        __syncthreads(); // V read out before P's top rows take Y
         // Y rows 0..b-1 into P (unit lower L)
         for (int idx = tid; idx < BB; idx += kCQThreads) {
             const int r = idx % B, j = idx / B;
@@ -893,10 +937,25 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
     }
     if (!grid_barrier_wt(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
     if (g != 0) {
-        for (int idx = tid; idx < BB; idx += kCQThreads) {
-            const int i = idx % B, j = idx / B;
-            M3[i + j * LD] = ld_wt(a.bc + idx);
-            M1[i + j * LD] = ld_wt(a.bc + BB + idx);
+        // U^-1 and T: rounds of 2 x LB loads per thread, all issued before their LDS stores
+        constexpr int LB = 8;
+        for (int base = 0; base < BB; base += LB * kCQThreads) {
+            double u[LB], t[LB];
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const int idx = base + q * kCQThreads + tid;
+                u[q] = (idx < BB) ? ld_wt(a.bc + idx) : 0.0;
+                t[q] = (idx < BB) ? ld_wt(a.bc + BB + idx) : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const int idx = base + q * kCQThreads + tid;
+                const int i = idx % B, j = idx / B;
+                if (idx < BB) {
+                    M3[i + j * LD] = u[q];
+                    M1[i + j * LD] = t[q];
+                }
+            }
         }
     }
     __syncthreads();

@@ -24,6 +24,7 @@ struct State {
     int on = -1; // -1: not yet read from the environment
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
+    std::vector<std::string> only; // tag prefixes to record; empty: every tag
 };
 
 State &st() {
@@ -56,10 +57,18 @@ bool enabled() {
     return S.on == 1;
 }
 
+bool selected(const State &S, const char *tag) {
+    if (S.only.empty()) return true;
+    for (const std::string &p : S.only)
+        if (std::strncmp(tag, p.c_str(), p.size()) == 0) return true;
+    return false;
+}
+
 Scope::Scope(hipStream_t s, const char *tag, double flops, double bytes) {
     if (!enabled()) return;
     State &S = st();
     std::lock_guard<std::mutex> lk(S.mu);
+    if (!selected(S, tag)) return;
     Rec r;
     r.tag = tag;
     r.a = take(S);
@@ -91,6 +100,24 @@ int cuev_profile_enable(int on) {
     State &S = st();
     std::lock_guard<std::mutex> lk(S.mu);
     S.on = on ? 1 : 0;
+    return CUEV_STATUS_SUCCESS;
+}
+
+int cuev_profile_select(const char *tags) {
+    State &S = st();
+    std::lock_guard<std::mutex> lk(S.mu);
+    S.only.clear();
+    if (!tags) return CUEV_STATUS_SUCCESS;
+    std::string cur;
+    for (const char *c = tags;; ++c) {
+        if (*c == ',' || *c == 0) {
+            if (!cur.empty()) S.only.push_back(cur);
+            cur.clear();
+            if (*c == 0) break;
+        } else if (*c != ' ') {
+            cur += *c;
+        }
+    }
     return CUEV_STATUS_SUCCESS;
 }
 

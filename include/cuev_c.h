@@ -310,6 +310,12 @@ int cuev_ssyr2k_lower(hipStream_t stream, int64_t n, int64_t k, float alpha, con
 /* CUEV_PROFILE=1 in the environment.                                        */
 /* ------------------------------------------------------------------------ */
 int cuev_profile_enable(int on);
+/* Restricts recording to tags that start with one of the comma-separated
+   prefixes in tags ("bc.chase,bt.gemm"); NULL or "" records every tag. Each
+   recorded launch adds two event records to its stream, which separate
+   back-to-back kernels by several microseconds: select the few tags a
+   measurement needs when the rest of the stream is being timed. */
+int cuev_profile_select(const char *tags);
 /* Waits for the recorded events and writes per-tag totals as JSON,
    {"tag": {"ms", "count", "flops", "bytes", "max_ms"}, ...}, into buf (len bytes).
    Returns the length the JSON needs (including the terminating NUL).
