@@ -87,12 +87,20 @@ template <typename T> int dbbr_reduce(Workspace<T> &ws, T *A, T *Bp) {
 
             if (w > 0) {
                 prof::Scope sc(s, "dbbr.corr", 8.0 * R * Bw * w);
-                // D = [Z_blk^T W ; Y_blk^T W]   (2w x b, ld 2w)
+                // D = [Z_blk Y_blk]^T W = [Z_blk^T W ; Y_blk^T W]   (2w x b, ld 2w): one product
+                // whose transposed A is split along its rows between the two blocks (the split
+                // must be a multiple of 64), else two
                 const i64 ldd = 2 * w;
-                CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Zb, lda, ws.W + off, lda, zero, ws.Dwk,
-                                 ldd, gs));
-                CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Yb, lda, ws.W + off, lda, zero,
-                                 ws.Dwk + w, ldd, gs));
+                if (w % 64 == 0) {
+                    CUEV_TRY(gemm_ex<T>(s, 2 * w, b, rows, one, opnd_cat(Zb, lda, Yb, lda, w, 1),
+                                        opnd(static_cast<const T *>(ws.W + off), lda, 0), zero, ws.Dwk, ldd,
+                                        0, gs));
+                } else {
+                    CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Zb, lda, ws.W + off, lda, zero, ws.Dwk,
+                                     ldd, gs));
+                    CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Yb, lda, ws.W + off, lda, zero,
+                                     ws.Dwk + w, ldd, gs));
+                }
                 // P -= [Y_blk | Z_blk] D
                 CUEV_TRY(gemm_ex<T>(s, rows, b, 2 * w, neg1, opnd_cat(Yb, lda, Zb, lda, w, 0),
                                     opnd(static_cast<const T *>(ws.Dwk), ldd, 0), one, P, lda, 0,

@@ -114,7 +114,10 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
 
 // Operand modes, fixed at compile time so that each kernel carries only the
 // loaders it needs (register pressure decides how many waves share a SIMD).
-enum AMode { A_N = 0, A_T = 1, A_SYM = 2, A_CAT = 3 }; // A_CAT: two m-contiguous sources
+// A_CAT: two m-contiguous sources split along k; A_TCAT: transposed, two k-contiguous
+// sources split along m (rows [0, split) of op(A) from a0, the rest from a1), with the
+// split a multiple of the tile height so that no tile straddles it
+enum AMode { A_N = 0, A_T = 1, A_SYM = 2, A_CAT = 3, A_TCAT = 4 };
 enum BMode { B_N = 0, B_T = 1, B_CATT = 2 };           // B_CATT: two n-contiguous sources
 
 // Source layouts of one R x BK operand tile (rows r = m or n of the product, k).
@@ -249,12 +252,19 @@ __device__ __forceinline__ void gemm_body_regs(const KParams<T> &p) {
         // ---- A
         {
             const T *pa = p.a0;
-            i64 lda = p.lda0, ko = 0;
+            i64 lda = p.lda0, ko = 0, mo = 0;
             bool fast = m_in && k_in && p.avec;
             if constexpr (AMODE == A_N) {
                 layA = L_RC;
             } else if constexpr (AMODE == A_T) {
                 layA = L_KC;
+            } else if constexpr (AMODE == A_TCAT) {
+                layA = L_KC;
+                if (m0 >= p.asplit) {
+                    pa = p.a1;
+                    lda = p.lda1;
+                    mo = p.asplit;
+                }
             } else if constexpr (AMODE == A_SYM) {
                 if (m0 >= k0 + BK - 1) {
                     layA = L_RC; // below the diagonal
@@ -272,8 +282,8 @@ __device__ __forceinline__ void gemm_body_regs(const KParams<T> &p) {
                     ko = p.asplit;
                 }
             }
-            if (fast) ra.load_vec(layA, pa, lda, m0, k0 - ko, tid);
-            else ra.load_scalar(layA, pa, lda, m0, k0 - ko, p.M, kend - ko, tid);
+            if (fast) ra.load_vec(layA, pa, lda, m0 - mo, k0 - ko, tid);
+            else ra.load_scalar(layA, pa, lda, m0 - mo, k0 - ko, p.M - mo, kend - ko, tid);
             if (layA == L_SYM) layA = L_RC; // stored with the RC chunk mapping
         }
         // ---- B (rows of the tile = n)
@@ -699,6 +709,7 @@ int launch_modes(hipStream_t s, const KParams<T> &p, int splits, int ntiles, int
     if (am == A_T && bm == B_T) return launch_cfg<T, A_T, B_T>(s, p, splits, ntiles, cfg);
     if (am == A_SYM && bm == B_N) return launch_cfg<T, A_SYM, B_N>(s, p, splits, ntiles, cfg);
     if (am == A_CAT && bm == B_N) return launch_cfg<T, A_CAT, B_N>(s, p, splits, ntiles, cfg);
+    if (am == A_TCAT && bm == B_N) return launch_one<T, 64, 64, 2, 2, 2, A_TCAT, B_N>(s, p, splits, ntiles);
     if (am == A_CAT && bm == B_CATT) return launch_cfg<T, A_CAT, B_CATT>(s, p, splits, ntiles, cfg);
     CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "gemm: operand mode combination (%d, %d) not implemented", am, bm);
 }
@@ -767,7 +778,11 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     // a tall-skinny update (the band reduction's panel-wide products): 64 x 64 tiles
     // for the row parallelism
     const bool tall_skinny = !lower && !A.sym && N <= 64 && M >= 2048 && K < 1024;
-    if (tall_skinny || scratch.small_tiles) {
+    const bool tcat = A.split >= 0 && A.trans;
+    if (tcat && (A.split % 64 != 0 || B.split >= 0 || B.trans || A.sym))
+        CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED,
+                  "gemm: a transposed A split along m needs the split a multiple of 64 and B plain");
+    if (tall_skinny || scratch.small_tiles || tcat) {
         cfg = C64x64;
     } else if (lower) {
         cfg = (N >= 1024) ? C128x128 : C64x64;
@@ -821,12 +836,11 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     if (splits >= 1) {
         int am, bm;
         if (A.sym) am = A_SYM;
-        else if (A.split >= 0) am = A_CAT;
+        else if (A.split >= 0) am = A.trans ? A_TCAT : A_CAT;
         else am = A.trans ? A_T : A_N;
         if (B.split >= 0) bm = B_CATT;
         else bm = B.trans ? B_T : B_N;
-        if ((A.sym && (A.split >= 0 || A.trans)) || (A.split >= 0 && A.trans) ||
-            (B.split >= 0 && !B.trans) || B.sym)
+        if ((A.sym && (A.split >= 0 || A.trans)) || (B.split >= 0 && !B.trans) || B.sym)
             CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "gemm: operand mode combination not implemented");
         CUEV_TRY(launch_modes<T>(s, p, splits, (int)ntiles, cfg, am, bm));
     }
