@@ -182,14 +182,14 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDesc *md, co
     for (int it = 0; it < 100; ++it) {
         T psi = 0, dpsi = 0, phi = 0, dphi = 0;
         for (int i = lane; i < k; i += 64) {
-            const T dlt = (D[i] - Do) - t;
-            const T q = W[i] * W[i] / dlt;
+            const T r = T(1) / ((D[i] - Do) - t); // one division per term
+            const T q = W[i] * W[i] * r;
             if (i <= j) {
                 psi += q;
-                dpsi += q / dlt;
+                dpsi += q * r;
             } else {
                 phi += q;
-                dphi += q / dlt;
+                dphi += q * r;
             }
         }
         psi = wave_sum(psi);
@@ -243,13 +243,15 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDesc *md, co
     }
 }
 
-// Gu-Eisenstat zhat, one thread per pole
+// Gu-Eisenstat zhat, one wave per pole: lane l multiplies the factors j = l, l + 64, ...
+// and the 64 partial products are multiplied in a fixed butterfly order
 template <typename T>
 __global__ __launch_bounds__(256) void dc_zhat_kernel(const MergeDesc *md, const int *rmerge,
                                                       const T *dl, const T *w, const int *orig,
                                                       const T *tau, T *zh, int nroots) {
-    const int gi = blockIdx.x * 256 + threadIdx.x;
-    if (gi >= nroots) return;
+    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (gi >= nroots) return; // uniform per wave
     const MergeDesc m = md[rmerge[gi]];
     const int base = m.root_begin, k = m.k, i = gi - base;
     const T *D = dl + base;
@@ -258,11 +260,13 @@ __global__ __launch_bounds__(256) void dc_zhat_kernel(const MergeDesc *md, const
     const T di = D[i];
     const T rho = (T)m.rho;
     T p = 1;
-    for (int j = 0; j < k; ++j) {
+    for (int j = lane; j < k; j += 64) {
         const T lmd = -((di - D[O[j]]) - TA[j]); // lambda_j - d_i
         p *= (j == i) ? lmd / rho : lmd / (D[j] - di);
     }
-    zh[gi] = copysign(sqrt(fmax(p, T(0))), w[gi]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) p *= __shfl_xor(p, off, 64);
+    if (lane == 0) zh[gi] = copysign(sqrt(fmax(p, T(0))), w[gi]);
 }
 
 // S columns: S[p, j] = zhat_i / (d_i - lambda_j) normalised, p the grouped
@@ -813,7 +817,7 @@ int dc_solve(DcState<T> &st, hipStream_t s, int n, const T *d, const T *e, T *Q,
             }
             {
                 prof::Scope sc(s, "dc.zhat_s");
-                hipLaunchKernelGGL(dc_zhat_kernel<T>, dim3((unsigned)ceil_div(nroot, 256)), dim3(256),
+                hipLaunchKernelGGL(dc_zhat_kernel<T>, dim3((unsigned)ceil_div(nroot, 4)), dim3(256),
                                    0, s, d_md, d_rm, d_dl, d_w, st.d_orig, st.d_tau, st.d_zh, nroot);
                 CUEV_LAUNCH_CHECK();
                 hipLaunchKernelGGL(dc_s_kernel<T>, dim3((unsigned)nroot), dim3(256), 0, s, d_md, d_rm,
