@@ -355,10 +355,13 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
 // Per pass: partial Gram -> global, grid barrier, each workgroup sums a slice of
 // the entries over all partials in a fixed order, grid barrier, then every
 // workgroup factors the same b x b Gram matrix redundantly (bit-identical
-// results everywhere). Workgroup 0 then factors Q_1 and publishes U^-1 and T
-// (fifth barrier). The b x b factorizations keep the matrix in registers as 4x4
-// blocks per thread and broadcast the pivot column/row through LDS: one LDS
-// barrier per elimination step. All arithmetic is FP64.
+// results everywhere) and solves its rows against it (P <- P R_p^-1, one thread
+// per row). Workgroup 0 then factors Q_1 and publishes U and Mw = -S L^-T (fifth
+// barrier): every workgroup forms its rows of W = Y T = (Q - [S; 0]) Mw and of
+// Y = Q_2 U^-1 (again by substitution along its rows), and workgroup 0 forms
+// T = U Mw. The b x b factorizations keep the matrix in registers as 4x4 blocks
+// per thread and broadcast the pivot column/row through LDS: one LDS barrier per
+// elimination step. All arithmetic is FP64.
 constexpr int kCQThreads = 256;
 constexpr int kCQMaxCH = 96;
 
@@ -827,48 +830,30 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
             a.A[(i64)r + (i64)j * a.lda] = (T)v;
             if (a.Y) a.Y[(i64)r + (i64)j * a.ldy] = (T)((j < r) ? Lm[r + j * LD] : (j == r ? 1.0 : 0.0));
         }
-        // L^-1 into P rows 0..b-1 (Q_1 is no longer needed there)
+        // Mw = -S L^-T (upper triangular: Mw(k, j) = -s_k L^-1(j, k)) into M1 and, with U^-1,
+        // into bc for the other workgroups. Every workgroup then forms its rows of
+        // W = Y T = (Q - [S; 0]) Mw from its rows of Q (T = -U S L^-T gives
+        // Q_2 U^-1 T = -Q_2 S L^-T below the first b rows, and L T = -(Q_1 - S) S L^-T on
+        // them) and Y = Q_2 U^-1; workgroup 0 also forms T = U Mw for the output, after the
+        // barrier.
+        __syncthreads(); // R in M1 and L in M2 read above
         if (own)
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) P[(size_t)(4 * tj + v) * CHp + 4 * ti + u] = xr[u][v];
-        __syncthreads();
-        // T = -U S L^-T  (T(i, j) = -sum_k U(i, k) s_k Linv(j, k)) into M1
-        {
-            double acc[4][4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-            if (own)
-                for (int k = 0; k < B; ++k) {
-                    const double sk = sv[k];
-                    double x[4], w[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        x[u] = Um[(si + NB * u) + k * LD] * sk;
-                        w[u] = P[(size_t)k * CHp + sj + NB * u];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) acc[u][v] -= x[u] * w[v];
+                for (int v = 0; v < 4; ++v) {
+                    const int j = 4 * ti + u, k = 4 * tj + v; // xr[u][v] = L^-1(j, k)
+                    const double m = -sv[k] * xr[u][v];
+                    M1[k + j * LD] = m;
+                    st_wt(a.bc + BB + k + (size_t)j * B, m);
                 }
-            if (own)
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) M1[(si + NB * u) + (sj + NB * v) * LD] = acc[u][v];
-        }
-        // U^-1 by recursive doubling, into P's top B rows (V(i, j) at V[j * CHp + i]; L^-1
-        // is no longer needed there once T is formed): each of the B/4 diagonal 4 x 4 blocks
-        // by one thread, then merges of two inverted diagonal blocks of size h into one of
-        // size 2h, V12 = -V11 (U12 V22). X = U12 V22 is staged in xs (the 8B doubles from
-        // cb to gd, free here) in chunks of at most 16 columns. Only the upper triangles
-        // of the diagonal blocks are read, so V's lower part is never formed.
-        __syncthreads(); // the T product is done with P's top rows
-        double *V = P;
+        // U^-1 by recursive doubling, into M2 (V(i, j) at V[i + j * LD]; L is written out
+        // above): each of the B/4 diagonal 4 x 4 blocks by one thread, then merges of two
+        // inverted diagonal blocks of size h into one of size 2h, V12 = -V11 (U12 V22).
+        // X = U12 V22 is staged in xs (the 8B doubles from cb to gd, free here) in chunks
+        // of at most 16 columns. Only the upper triangles of the diagonal blocks are read;
+        // the lower triangle is cleared afterwards.
+        double *V = M2;
         double *xs = cb;
         if (tid < B / 4) {
             const int c0 = 4 * tid;
@@ -890,7 +875,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     v[i][j] = -t * rd[i];
                 }
 #pragma unroll
-                for (int i = 0; i <= j; ++i) V[(size_t)(c0 + j) * CHp + c0 + i] = v[i][j];
+                for (int i = 0; i <= j; ++i) V[(c0 + i) + (c0 + j) * LD] = v[i][j];
             }
         }
         for (int h = 4; h < B; h *= 2) {
@@ -905,7 +890,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     double t4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
                     for (int k = 0; k <= j; ++k)
-                        t4[k & 3] += Um[(c + i) + (c + h + k) * LD] * V[(size_t)(c + h + j) * CHp + c + h + k];
+                        t4[k & 3] += Um[(c + i) + (c + h + k) * LD] * V[(c + h + k) + (c + h + j) * LD];
                     xs[rr + jj * HR] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
                 }
                 __syncthreads();
@@ -914,30 +899,24 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     const int c = 2 * h * q, j = j0 + jj;
                     double t4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
-                    for (int k = i; k < h; ++k) t4[k & 3] += V[(size_t)(c + k) * CHp + c + i] * xs[(q * h + k) + jj * HR];
-                    V[(size_t)(c + h + j) * CHp + c + i] = -((t4[0] + t4[1]) + (t4[2] + t4[3]));
+                    for (int k = i; k < h; ++k) t4[k & 3] += V[(c + i) + (c + k) * LD] * xs[(q * h + k) + jj * HR];
+                    V[(c + i) + (c + h + j) * LD] = -((t4[0] + t4[1]) + (t4[2] + t4[3]));
                 }
             }
         }
         __syncthreads();
         for (int idx = tid; idx < BB; idx += kCQThreads) {
             const int i = idx % B, j = idx / B;
-            const double v = (i <= j) ? V[(size_t)j * CHp + i] : 0.0;
-            M3[i + j * LD] = v;
+            const double v = (i <= j) ? V[i + j * LD] : 0.0;
+            V[i + j * LD] = v;
             st_wt(a.bc + idx, v);
         }
-        for (int idx = tid; idx < BB; idx += kCQThreads) st_wt(a.bc + BB + idx, M1[(idx % B) + (idx / B) * LD]);
-        __syncthreads(); // V read out before P's top rows take Y
-        // Y rows 0..b-1 into P (unit lower L)
-        for (int idx = tid; idx < BB; idx += kCQThreads) {
-            const int r = idx % B, j = idx / B;
-            P[(size_t)j * CHp + r] = (j < r) ? Lm[r + j * LD] : (j == r ? 1.0 : 0.0);
-        }
+        if (tid < B) P[(size_t)tid * CHp + tid] -= sv[tid]; // Q - [S; 0] on the top block
         if (tid == 0) *a.gate = 0;
     }
     if (!grid_barrier_wt(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
     if (g != 0) {
-        // U^-1 and T: rounds of 2 x LB loads per thread, all issued before their LDS stores
+        // U^-1 and Mw: rounds of 2 x LB loads per thread, all issued before their LDS stores
         constexpr int LB = 8;
         for (int base = 0; base < BB; base += LB * kCQThreads) {
             double u[LB], t[LB];
@@ -957,20 +936,9 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                 }
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
-    // Y = Q U^-1 for the rows below the first b
-    const int ylo = (g == 0) ? B : 0;
-    cq_trmm<B>(P, CHp, ylo, CH, M3, tid);
-    // outputs: Y, the reflector tails in A, and W = Y T
-    for (int idx = tid; idx < B * chl; idx += kCQThreads) {
-        const int j = idx / chl, rl = idx - j * chl;
-        if (rl < ylo) continue;
-        const T y = (T)P[(size_t)j * CHp + rl];
-        const i64 r = r0g + rl;
-        a.A[r + (i64)j * a.lda] = y;
-        if (a.Y) a.Y[r + (i64)j * a.ldy] = y;
-    }
+    // W = (Q - [S; 0]) Mw for every row of this workgroup, before Y replaces Q
     if (a.W) {
         const int nrb = (chl + 3) / 4;
         for (int blk = tid; blk < nrb * NB; blk += kCQThreads) {
@@ -981,7 +949,7 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
             for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-            for (int i = 0; i <= cj + NB * 3; ++i) {
+            for (int i = 0; i <= cj + NB * 3; ++i) { // Mw(i, j) = 0 for i > j
                 double x[4], w[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) x[u] = P[(size_t)i * CHp + rr + u];
@@ -999,13 +967,34 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     for (int v = 0; v < 4; ++v)
                         a.W[(i64)(r0g + rr + u) + (i64)(cj + NB * v) * a.ldw] = (T)acc[u][v];
         }
+        __syncthreads();
+    }
+    // Y = Q U^-1 for the rows below the first b (workgroup 0 wrote its top rows, L, and
+    // holds U^-1 in M2 and U in M3)
+    const int ylo = (g == 0) ? B : 0;
+    cq_trmm<B>(P, CHp, ylo, CH, (g == 0) ? M2 : M3, tid);
+    // outputs: Y and the reflector tails in A
+    for (int idx = tid; idx < B * chl; idx += kCQThreads) {
+        const int j = idx / chl, rl = idx - j * chl;
+        if (rl < ylo) continue;
+        const T y = (T)P[(size_t)j * CHp + rl];
+        const i64 r = r0g + rl;
+        a.A[r + (i64)j * a.lda] = y;
+        if (a.Y) a.Y[r + (i64)j * a.ldy] = y;
     }
     if (g == 0) {
-        for (int idx = tid; idx < BB; idx += kCQThreads) {
-            const int i = idx % B, j = idx / B;
-            a.Tm[idx] = (T)M1[i + j * LD];
-        }
-        for (int j = tid; j < B; j += kCQThreads) a.tau[j] = (T)M1[j + j * LD];
+        // T = U Mw (upper triangular), and tau = diag(T)
+        double acc[4][4];
+        cq_mm<B, false>(acc, M3, M1, si, sj);
+        if (own)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = si + NB * u, j = sj + NB * v;
+                    a.Tm[i + (size_t)j * B] = (T)acc[u][v];
+                    if (i == j) a.tau[j] = (T)acc[u][v];
+                }
     }
 }
 

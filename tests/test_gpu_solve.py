@@ -178,6 +178,37 @@ def test_timer_fields(torch_dev):
     assert t["total_ms"] >= t["dc_ms"]
 
 
+def test_profile_select(torch_dev):
+    """The event profiler records every launch site by default and only the selected
+    tag prefixes after cuev_profile_select; the selected sites' totals do not depend on
+    the selection, and the solve's results do not depend on the profiler."""
+    torch = torch_dev
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((2048, 2048))
+    A = (G + G.T) / 2
+    try:
+        lam0, V0 = solve_np(torch, A, bt_mode="reordered")
+        cv.profile_enable(True)
+        cv.profile_read(reset=True)
+        lam1, V1 = solve_np(torch, A, bt_mode="reordered")
+        every = cv.profile_read(reset=True)
+        cv.profile_enable(True, only=("bc.chase", "bt.gemm"))
+        lam2, V2 = solve_np(torch, A, bt_mode="reordered")
+        some = cv.profile_read(reset=True)
+        cv.profile_enable(True, only="dbbr.")
+        solve_np(torch, A, bt_mode="reordered")
+        dbbr = cv.profile_read(reset=True)
+    finally:
+        cv.profile_enable(False)
+    assert {"bc.chase", "bt.gemm", "dbbr.symm", "dbbr.panel_qr", "dc.gemm"} <= set(every)
+    assert set(some) == {"bc.chase", "bt.gemm"}
+    assert set(dbbr) and all(t.startswith("dbbr.") for t in dbbr)
+    for t in ("bc.chase", "bt.gemm"):
+        assert some[t]["count"] == every[t]["count"] and some[t]["flops"] == every[t]["flops"]
+    assert np.array_equal(lam0, lam1) and np.array_equal(lam0, lam2)
+    assert np.array_equal(V0, V1) and np.array_equal(V0, V2)
+
+
 def test_reordered_repeatable_n16384(torch_dev):
     """The full pipeline at n=16384 in the reordered order, where the bulge chasing
     runs beside the Q_s^T back-transform: eight solves of the same matrix must give
