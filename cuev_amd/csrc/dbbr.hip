@@ -36,6 +36,25 @@ __global__ __launch_bounds__(256) void bc_pack_kernel(const T *A, i64 lda, T *Bp
     Bp[idx] = (r <= b && i < n) ? A[i + j * lda] : T(0);
 }
 
+// The b x b block G of D (rows [w, w + b), ld ldd) holds G = P^T W. With E = D_y^T D_z
+// (b x b, ld b), C = W^T (P - Y_blk D_z - Z_blk D_y) = G^T - E - E^T, and G becomes C / 2.
+// One thread per unordered pair (r, c), r <= c, which reads and writes both (r, c) and
+// (c, r), so no entry is read after another thread has written it.
+template <typename T>
+__global__ __launch_bounds__(256) void dbbr_c_half_kernel(T *G, i64 ldd, const T *E, int b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b * (b + 1) / 2) return;
+    int c = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5); // t = c (c + 1) / 2 + r, r <= c
+    while ((c + 1) * (c + 2) / 2 <= t) ++c;
+    while (c * (c + 1) / 2 > t) --c;
+    const int r = t - c * (c + 1) / 2;
+    const T grc = G[r + (i64)c * ldd], gcr = G[c + (i64)r * ldd];
+    const T erc = E[r + (i64)c * b], ecr = E[c + (i64)r * b];
+    // C(r, c) = G(c, r) - E(r, c) - E(c, r); C(c, r) = G(r, c) - E(c, r) - E(r, c)
+    G[r + (i64)c * ldd] = T(0.5) * (gcr - erc - ecr);
+    G[c + (i64)r * ldd] = T(0.5) * (grc - ecr - erc);
+}
+
 } // namespace
 
 template <typename T> int dbbr_pack(Workspace<T> &ws, const T *A, T *Bp, int n, int b) {
@@ -85,35 +104,57 @@ template <typename T> int dbbr_reduce(Workspace<T> &ws, T *A, T *Bp) {
                 CUEV_TRY(symm_lower<T>(s, rows, b, one, A + tr, lda, ws.W + off, lda, zero, P, lda, gs));
             }
 
-            if (w > 0) {
-                prof::Scope sc(s, "dbbr.corr", 8.0 * R * Bw * w);
-                // D = [Z_blk Y_blk]^T W = [Z_blk^T W ; Y_blk^T W]   (2w x b, ld 2w): one product
-                // whose transposed A is split along its rows between the two blocks (the split
-                // must be a multiple of 64), else two
-                const i64 ldd = 2 * w;
-                if (w % 64 == 0) {
-                    CUEV_TRY(gemm_ex<T>(s, 2 * w, b, rows, one, opnd_cat(Zb, lda, Yb, lda, w, 1),
-                                        opnd(static_cast<const T *>(ws.W + off), lda, 0), zero, ws.Dwk, ldd,
-                                        0, gs));
-                } else {
-                    CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Zb, lda, ws.W + off, lda, zero, ws.Dwk,
-                                     ldd, gs));
-                    CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Yb, lda, ws.W + off, lda, zero,
-                                     ws.Dwk + w, ldd, gs));
+            if (w > 0 && w % 64 == 0 && b % 64 == 0) {
+                // Fused form. One reduction over the rows gives D = [Z_blk P Y_blk]^T W =
+                // [D_z; G; D_y] ((2w + b) x b, ld 2w + b; [Z_blk | P] are adjacent columns of Z),
+                // E = D_y^T D_z, and G becomes C / 2 with
+                // C = W^T (P - Y_blk D_z - Z_blk D_y) = G^T - E - E^T; one update then gives
+                // Z = P - [Y_blk Y_cur Z_blk] [D_z; C/2; D_y] ([Y_blk | Y_cur] are adjacent
+                // columns of Y). Same algebra as the two steps below, with two products fewer.
+                prof::Scope sc(s, "dbbr.corr", 4.0 * R * Bw * (2.0 * w + Bw) + 2.0 * Bw * Bw * w);
+                const i64 ldd = 2 * w + b;
+                T *E = ws.Dwk + (i64)(2 * k - b) * b; // past the (2w + b) x b block: 2w + b <= 2k - b
+                CUEV_TRY(gemm_ex<T>(s, 2 * w + b, b, rows, one, opnd_cat(Zb, lda, Yb, lda, w + b, 1),
+                                    opnd(static_cast<const T *>(ws.W + off), lda, 0), zero, ws.Dwk, ldd,
+                                    0, gs));
+                CUEV_TRY(gemm<T>(s, 1, 0, b, b, w, one, ws.Dwk + w + b, ldd, ws.Dwk, ldd, zero, E, b, gs));
+                hipLaunchKernelGGL(dbbr_c_half_kernel<T>, dim3((unsigned)ceil_div((i64)b * (b + 1) / 2, 256)),
+                                   dim3(256), 0, s, ws.Dwk + w, ldd, static_cast<const T *>(E), b);
+                CUEV_LAUNCH_CHECK();
+                CUEV_TRY(gemm_ex<T>(s, rows, b, 2 * w + b, neg1, opnd_cat(Yb, lda, Zb, lda, w + b, 0),
+                                    opnd(static_cast<const T *>(ws.Dwk), ldd, 0), one, P, lda, 0, gs));
+            } else {
+                if (w > 0) {
+                    prof::Scope sc(s, "dbbr.corr", 8.0 * R * Bw * w);
+                    // D = [Z_blk Y_blk]^T W = [Z_blk^T W ; Y_blk^T W]   (2w x b, ld 2w): one product
+                    // whose transposed A is split along its rows between the two blocks (the split
+                    // must be a multiple of 64), else two
+                    const i64 ldd = 2 * w;
+                    if (w % 64 == 0) {
+                        CUEV_TRY(gemm_ex<T>(s, 2 * w, b, rows, one, opnd_cat(Zb, lda, Yb, lda, w, 1),
+                                            opnd(static_cast<const T *>(ws.W + off), lda, 0), zero, ws.Dwk, ldd,
+                                            0, gs));
+                    } else {
+                        CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Zb, lda, ws.W + off, lda, zero, ws.Dwk,
+                                         ldd, gs));
+                        CUEV_TRY(gemm<T>(s, 1, 0, w, b, rows, one, Yb, lda, ws.W + off, lda, zero,
+                                         ws.Dwk + w, ldd, gs));
+                    }
+                    // P -= [Y_blk | Z_blk] D
+                    CUEV_TRY(gemm_ex<T>(s, rows, b, 2 * w, neg1, opnd_cat(Yb, lda, Zb, lda, w, 0),
+                                        opnd(static_cast<const T *>(ws.Dwk), ldd, 0), one, P, lda, 0,
+                                        gs));
                 }
-                // P -= [Y_blk | Z_blk] D
-                CUEV_TRY(gemm_ex<T>(s, rows, b, 2 * w, neg1, opnd_cat(Yb, lda, Zb, lda, w, 0),
-                                    opnd(static_cast<const T *>(ws.Dwk), ldd, 0), one, P, lda, 0,
-                                    gs));
+
+                // C = W^T P  (b x b) ; Z = P - 1/2 Y C
+                {
+                    prof::Scope sc(s, "dbbr.wtp_z", 4.0 * R * Bw * Bw);
+                    CUEV_TRY(gemm<T>(s, 1, 0, b, b, rows, one, ws.W + off, lda, P, lda, zero, ws.Dwk, b, gs));
+                    CUEV_TRY(gemm<T>(s, 0, 0, rows, b, b, neg_half, ws.Y + off, lda, ws.Dwk, b, one, P, lda,
+                                     gs));
+                }
             }
 
-            // C = W^T P  (b x b) ; Z = P - 1/2 Y C
-            {
-                prof::Scope sc(s, "dbbr.wtp_z", 4.0 * R * Bw * Bw);
-                CUEV_TRY(gemm<T>(s, 1, 0, b, b, rows, one, ws.W + off, lda, P, lda, zero, ws.Dwk, b, gs));
-                CUEV_TRY(gemm<T>(s, 0, 0, rows, b, b, neg_half, ws.Y + off, lda, ws.Dwk, b, one, P, lda,
-                                 gs));
-            }
             w += b;
 
             // bring the next panel's columns up to date with the whole block so far:

@@ -804,10 +804,11 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     p.tiles_m = (int)ceil_div(M, BMs);
     p.tiles_n = (int)ceil_div(N, BNs);
 
-    // split-K for products with few output tiles and a long K. The K chunks are
-    // short (64): a workgroup of these products waits on memory latency at every
-    // K step, so many short chunks keep more loads in flight than a few long ones.
-    // Tall-skinny updates (N <= 64, short K) are split four ways.
+    // split-K for products with few output tiles and a long K (from 1024, or from 256
+    // for at most four tiles). The K chunks are short (64): a workgroup of these
+    // products waits on memory latency at every K step, so many short chunks keep more
+    // loads in flight than a few long ones. Tall-skinny updates (N <= 64, short K) are
+    // split four ways.
     int splits = 1;
     const i64 target = (sym_panel ? 16LL : 2LL) * cus;
     const i64 kmin = sym_panel ? 256 : 64; // shortest K chunk per split
@@ -817,7 +818,7 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
         const i64 cap = (i64)(scratch.bytes / ((size_t)M * N * sizeof(T)));
         const i64 want = std::min<i64>(std::min<i64>(4, K / 64), cap);
         if (want > 1) splits = (int)want;
-    } else if (ntiles < target / 2 && K >= 1024) {
+    } else if (ntiles < target / 2 && (K >= 1024 || (ntiles <= 4 && K >= 256))) {
         i64 want = std::min<i64>(ceil_div(target, ntiles), K / kmin);
         const i64 cap = (i64)(scratch.bytes / ((size_t)M * N * sizeof(T)));
         want = std::min(want, cap);

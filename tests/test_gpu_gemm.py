@@ -20,6 +20,8 @@ CASES = [
     (250, 130, 70, 0, 1),
     (99, 77, 300, 1, 1),
     (64, 64, 5000, 1, 0),      # split-K: b x b = W^T P
+    (64, 64, 448, 1, 0),       # split-K from K = 256 for few tiles: E = D_y^T D_z
+    (130, 70, 260, 0, 0),      # the same with ragged tiles and a K tail
     (2000, 64, 2000, 0, 0),    # tall-skinny: P = A22 W
     (64, 1500, 3000, 1, 0),    # Y^T M in sbr_back
     (1024, 1024, 1024, 0, 0),
