@@ -355,11 +355,10 @@ template <typename T> __global__ __launch_bounds__(kPQThreads) void panel_qr_ker
 // Per pass: partial Gram -> global, grid barrier, each workgroup sums a slice of
 // the entries over all partials in a fixed order, grid barrier, then every
 // workgroup factors the same b x b Gram matrix redundantly (bit-identical
-// results everywhere) and solves its rows against it (P <- P R_p^-1, one thread
-// per row). Workgroup 0 then factors Q_1 and publishes U and Mw = -S L^-T (fifth
-// barrier): every workgroup forms its rows of W = Y T = (Q - [S; 0]) Mw and of
-// Y = Q_2 U^-1 (again by substitution along its rows), and workgroup 0 forms
-// T = U Mw. The b x b factorizations keep the matrix in registers as 4x4 blocks
+// results everywhere), with X = L'^-1 alongside for P <- P R_p^-1. Workgroup 0 then
+// factors Q_1 and publishes U and Mw = -S L^-T (fifth barrier): every workgroup forms
+// its rows of W = Y T = (Q - [S; 0]) Mw and of Y = Q_2 U^-1 (by substitution along
+// its rows, four lanes per row), and workgroup 0 forms T = U Mw. The b x b factorizations keep the matrix in registers as 4x4 blocks
 // per thread and broadcast the pivot column/row through LDS: one LDS barrier per
 // elimination step. All arithmetic is FP64.
 constexpr int kCQThreads = 256;
@@ -486,6 +485,49 @@ __device__ __forceinline__ void cq_trmm(double *P, int CHp, int r0, int r1, cons
                     if (rr + u < r1) P[(size_t)(tj + NB * v) * CHp + rr + u] = acc[u][v];
         __syncthreads();
     }
+}
+
+// P[r, :] <- P[r, :] U^-1 for the rows r0 <= r < r1, in place: forward substitution along
+// each row, four lanes per row (a quad of one wave) and 64 rows per round. The columns go
+// in blocks of 16, lane q of a quad holding columns 16J + 4q .. 16J + 4q + 3 of block J:
+// the block's columns first take the contributions of the finished blocks (read back from
+// P), then are finished one by one, each broadcast within the quad by a lane shuffle. U is
+// upper triangular in LDS (column-major, ld B + 1) and rd[j] = 1 / U(j, j). A row's values
+// are written and read back by its own quad only, so no workgroup barrier is needed until
+// the end.
+template <int B>
+__device__ __forceinline__ void cq_rsolve(double *P, int CHp, int r0, int r1, const double *U, const double *rd,
+                                          int tid) {
+    static_assert(B % 16 == 0 && kCQThreads == 256, "row solve layout");
+    constexpr int LD = B + 1;
+    const int q = tid & 3, lane = tid & 63;
+    for (int rb = r0; rb < r1; rb += kCQThreads / 4) {
+        const int r = rb + (tid >> 2);
+        const bool act = r < r1;
+#pragma unroll 1
+        for (int J = 0; J < B / 16; ++J) {
+            const int jb = 16 * J + 4 * q; // this lane's first column of the block
+            double acc[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] = act ? P[(size_t)(jb + c) * CHp + r] : 0.0;
+            for (int k = 0; k < 16 * J; ++k) {
+                const double y = act ? P[(size_t)k * CHp + r] : 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] -= y * U[k + (jb + c) * LD];
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int qt = t >> 2, ct = t & 3, kt = 16 * J + t;
+                const double mine = (q == qt) ? acc[ct] * rd[kt] : 0.0;
+                const double y = __shfl(mine, (lane & ~3) + qt, 64);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (4 * q + c > t) acc[c] -= y * U[kt + (jb + c) * LD];
+                if (q == qt && act) P[(size_t)kt * CHp + r] = y;
+            }
+        }
+    }
+    __syncthreads();
 }
 
 template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel_cholqr_kernel(CQArgs<T> a) {
@@ -830,12 +872,12 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
             a.A[(i64)r + (i64)j * a.lda] = (T)v;
             if (a.Y) a.Y[(i64)r + (i64)j * a.ldy] = (T)((j < r) ? Lm[r + j * LD] : (j == r ? 1.0 : 0.0));
         }
-        // Mw = -S L^-T (upper triangular: Mw(k, j) = -s_k L^-1(j, k)) into M1 and, with U^-1,
+        // Mw = -S L^-T (upper triangular: Mw(k, j) = -s_k L^-1(j, k)) into M1 and, with U,
         // into bc for the other workgroups. Every workgroup then forms its rows of
         // W = Y T = (Q - [S; 0]) Mw from its rows of Q (T = -U S L^-T gives
         // Q_2 U^-1 T = -Q_2 S L^-T below the first b rows, and L T = -(Q_1 - S) S L^-T on
-        // them) and Y = Q_2 U^-1; workgroup 0 also forms T = U Mw for the output, after the
-        // barrier.
+        // them) and Y = Q_2 U^-1 by substitution along its rows; workgroup 0 also forms
+        // T = U Mw for the output, after the barrier.
         __syncthreads(); // R in M1 and L in M2 read above
         if (own)
 #pragma unroll
@@ -847,76 +889,13 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
                     M1[k + j * LD] = m;
                     st_wt(a.bc + BB + k + (size_t)j * B, m);
                 }
-        // U^-1 by recursive doubling, into M2 (V(i, j) at V[i + j * LD]; L is written out
-        // above): each of the B/4 diagonal 4 x 4 blocks by one thread, then merges of two
-        // inverted diagonal blocks of size h into one of size 2h, V12 = -V11 (U12 V22).
-        // X = U12 V22 is staged in xs (the 8B doubles from cb to gd, free here) in chunks
-        // of at most 16 columns. Only the upper triangles of the diagonal blocks are read;
-        // the lower triangle is cleared afterwards.
-        double *V = M2;
-        double *xs = cb;
-        if (tid < B / 4) {
-            const int c0 = 4 * tid;
-            double u[4][4], v[4][4], rd[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) u[i][j] = Um[(c0 + i) + (c0 + j) * LD];
-                rd[i] = 1.0 / u[i][i];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                v[j][j] = rd[j];
-#pragma unroll
-                for (int i = j - 1; i >= 0; --i) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int k = i + 1; k <= j; ++k) t += u[i][k] * v[k][j];
-                    v[i][j] = -t * rd[i];
-                }
-#pragma unroll
-                for (int i = 0; i <= j; ++i) V[(c0 + i) + (c0 + j) * LD] = v[i][j];
-            }
-        }
-        for (int h = 4; h < B; h *= 2) {
-            const int cw = h < 16 ? h : 16;
-            constexpr int HR = B / 2; // rows of X over all merges of a level
-            for (int j0 = 0; j0 < h; j0 += cw) {
-                __syncthreads(); // V22 (and the previous chunk's reads of xs) complete
-                for (int e = tid; e < HR * cw; e += kCQThreads) {
-                    const int rr = e % HR, jj = e / HR, q = rr / h, i = rr % h;
-                    const int c = 2 * h * q, j = j0 + jj;
-                    // four partial sums, so the LDS reads of consecutive k overlap
-                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-                    for (int k = 0; k <= j; ++k)
-                        t4[k & 3] += Um[(c + i) + (c + h + k) * LD] * V[(c + h + k) + (c + h + j) * LD];
-                    xs[rr + jj * HR] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-                }
-                __syncthreads();
-                for (int e = tid; e < HR * cw; e += kCQThreads) {
-                    const int rr = e % HR, jj = e / HR, q = rr / h, i = rr % h;
-                    const int c = 2 * h * q, j = j0 + jj;
-                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-                    for (int k = i; k < h; ++k) t4[k & 3] += V[(c + i) + (c + k) * LD] * xs[(q * h + k) + jj * HR];
-                    V[(c + i) + (c + h + j) * LD] = -((t4[0] + t4[1]) + (t4[2] + t4[3]));
-                }
-            }
-        }
-        __syncthreads();
-        for (int idx = tid; idx < BB; idx += kCQThreads) {
-            const int i = idx % B, j = idx / B;
-            const double v = (i <= j) ? V[i + j * LD] : 0.0;
-            V[i + j * LD] = v;
-            st_wt(a.bc + idx, v);
-        }
+        for (int idx = tid; idx < BB; idx += kCQThreads) st_wt(a.bc + idx, Um[(idx % B) + (idx / B) * LD]);
         if (tid < B) P[(size_t)tid * CHp + tid] -= sv[tid]; // Q - [S; 0] on the top block
         if (tid == 0) *a.gate = 0;
     }
     if (!grid_barrier_wt(a.counter, (unsigned)a.G * ++nbar, a.err)) return;
     if (g != 0) {
-        // U^-1 and Mw: rounds of 2 x LB loads per thread, all issued before their LDS stores
+        // U and Mw: rounds of 2 x LB loads per thread, all issued before their LDS stores
         constexpr int LB = 8;
         for (int base = 0; base < BB; base += LB * kCQThreads) {
             double u[LB], t[LB];
@@ -938,6 +917,8 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
         }
         __syncthreads();
     }
+    if (tid < B) gd[tid] = 1.0 / M3[tid + tid * LD]; // 1 / U(j, j)
+    __syncthreads();
     // W = (Q - [S; 0]) Mw for every row of this workgroup, before Y replaces Q
     if (a.W) {
         const int nrb = (chl + 3) / 4;
@@ -969,10 +950,9 @@ template <typename T, int B> __global__ __launch_bounds__(kCQThreads) void panel
         }
         __syncthreads();
     }
-    // Y = Q U^-1 for the rows below the first b (workgroup 0 wrote its top rows, L, and
-    // holds U^-1 in M2 and U in M3)
+    // Y = Q U^-1 for the rows below the first b (workgroup 0 wrote its top rows, L)
     const int ylo = (g == 0) ? B : 0;
-    cq_trmm<B>(P, CHp, ylo, CH, (g == 0) ? M2 : M3, tid);
+    cq_rsolve<B>(P, CHp, ylo, chl, M3, gd, tid);
     // outputs: Y and the reflector tails in A
     for (int idx = tid; idx < B * chl; idx += kCQThreads) {
         const int j = idx / chl, rl = idx - j * chl;
