@@ -75,7 +75,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned *counter, unsigned target,
 }
 
 // Write-through exchange for the CholQR kernel: every global access to data that
-// another workgroup of the same launch reads (Gram partials, their sums, U^-1 and T)
+// another workgroup of the same launch reads (Gram partials, their sums, U and Mw)
 // is an agent-scope relaxed atomic access. On gfx950 these carry the sc1 bit: the
 // store is written through past the XCD-private L2 and the load reads from the
 // device-coherent level. A grid barrier then needs neither an L2 write-back before
@@ -376,7 +376,7 @@ template <typename T> struct CQArgs {
     int rows, CH, G;
     double *part; // [G][b*b] partial Gram matrices
     double *gsum; // [b*b]
-    double *bc;   // [2*b*b]: U^-1, then T (column-major)
+    double *bc;   // [2*b*b]: U, then Mw = -S L^-T (column-major)
     int *gate;
     unsigned *counter;
     unsigned *reset; // the Householder kernel's barrier counter, zeroed for the launch after this one
