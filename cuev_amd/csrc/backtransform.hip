@@ -73,7 +73,7 @@ template <> struct MfmaBt<float> {
 // with W = -V T (TR: -V T^T), stored as pairs: value c of lane l at
 // ((c/2) 64 + l) 2 + c%2, so the apply kernel reads each pair with one 16-byte load.
 template <typename T, int B, bool TR>
-__global__ __launch_bounds__(kBTThreads) void bt_pack_kernel(const T *__restrict__ U, i64 ldu,
+__global__ __launch_bounds__(kBTThreads, 2) void bt_pack_kernel(const T *__restrict__ U, i64 ldu,
                                                              const i64 *__restrict__ toff,
                                                              int ngroups, i64 nblocks, int n,
                                                              T *__restrict__ ops) {
@@ -99,10 +99,21 @@ __global__ __launch_bounds__(kBTThreads) void bt_pack_kernel(const T *__restrict
         h = (int)(id - toff[g]);
     }
     const int s0 = g * NG;
-    for (int e = lane; e < NG * B; e += 64) {
-        const int j = e / B, k = e - (e / B) * B;
-        const int sw = s0 + j;
-        sv[wave][j][k] = (act && sw <= n - 3) ? U[(i64)h * B + k + (i64)sw * ldu] : T(0);
+    {
+        // the block's NG reflectors: every load issued before its LDS store (one round trip)
+        constexpr int PER = NG * B / 64;
+        static_assert(NG * B % 64 == 0, "bt_pack load layout");
+        T v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = lane + 64 * u, j = e / B, k = e - j * B, sw = s0 + j;
+            v[u] = (act && sw <= n - 3) ? U[(i64)h * B + k + (i64)sw * ldu] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = lane + 64 * u, j = e / B, k = e - j * B;
+            sv[wave][j][k] = v[u];
+        }
     }
     __syncthreads();
     // strictly upper Gram: N[i][j] = v_i . v_j with v_j shifted down by j - i rows
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(kBTThreads) void bt_pack_kernel(const T *__restrict
         const int d = 16 * (c >> 2) + MB::kidx(c & 3, q) - li;
         ob[(c >> 1) * 128 + (c & 1)] = (d >= 0 && d < B) ? sv[wave][li][d] : T(0);
     }
-#pragma unroll
+#pragma unroll 1
     for (int t = 0; t < NKB; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
