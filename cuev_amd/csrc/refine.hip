@@ -11,16 +11,26 @@
 //      refinement for symmetric eigenvalue decomposition", JJIAM 35, 2018,
 //      Algorithm 1):
 //        G = X^T X,  R = I - G,  S = X^T A X,  lam_i = s_ii / (1 - r_ii),
-//        delta = 2 (||S - diag(lam)||_2 + ||A||_2 ||R||_2),
-//        E_ij = (s_ij + lam_j r_ij) / (lam_j - lam_i)  if |lam_i - lam_j| > delta,
-//             = r_ij / 2                              otherwise,
+//        E_ij = (s_ij + lam_j r_ij) / (lam_j - lam_i)  for a separated pair,
+//             = r_ij / 2                              for a clustered one,
 //        X <- X + X E.
-//      The 2-norms are bounded by Frobenius norms (a larger delta only moves
-//      more pairs to the r_ij/2 branch). Each sweep first measures the
-//      residual ||A X - X diag(lam)||_F / (n ||A||_F) and ||X^T X - I||_F / n
-//      from the products it needs anyway and stops once both are below the
-//      tolerance. Pairs inside a cluster (|lam_i - lam_j| <= delta) are
-//      orthogonalised but not rotated within the cluster; they are counted.
+//      The paper separates pairs by one global bound,
+//      |lam_i - lam_j| > delta = 2 (||S - diag(lam)||_2 + ||A||_2 ||R||_2).
+//      Here each pair is tested on its own: it is separated when both first-order
+//      corrections |s_ij + lam_j r_ij| and |s_ij + lam_i r_ij|, plus the rounding
+//      level of S (n u ||A||_F), are at most tau = 1e-4 of |lam_j - lam_i|, so that
+//      the step's second-order error (below tau^2 of the pair's coupling) is gone
+//      after the next step. On a geometric spectrum the global bound (with its
+//      2-norms bounded by Frobenius norms) marks nearly every pair of an FP32 start
+//      as clustered; the pair test marks those whose coupling is comparable to
+//      their gap. Each sweep first measures the residual
+//      ||A X - X diag(lam)||_F / (n ||A||_F) and ||X^T X - I||_F / n from the
+//      products it needs anyway and stops once both are below the tolerance.
+//      After the first Newton step (which leaves clustered pairs orthogonalised
+//      but not rotated), one Rayleigh-Ritz step on the columns of the clustered
+//      pairs resolves them (the cluster treatment of Ogita and Aishima's
+//      refinement for clustered eigenvalues, JJIAM 36, 2019, applied to the union
+//      of the clusters), and the next Newton step finishes the separated pairs.
 //   All products run on the FP64 MFMA engine; the fused element-wise kernels
 //   below build lam, E and the norms.
 #include "refine.hpp"
@@ -49,8 +59,18 @@ namespace {
 
 constexpr int kRedBlocks = 1024; // fixed partial count: deterministic reductions
 // clustered columns from which the Rayleigh-Ritz step runs on the GPU (one
-// FP64 solve of the union) instead of host dsygvd per cluster
+// FP64 solve of the projected union) instead of one host dsygvd
 constexpr int kRRDeviceMin = 512;
+// a pair is separated when its first-order corrections are at most kPairTau of its gap
+constexpr double kPairTau = 1e-4;
+
+// the pair test: clustered unless both first-order corrections of the pair (i, j),
+// plus nu (the rounding level of S), are at most kPairTau |lam_j - lam_i|; a NaN
+// tests as clustered
+__device__ inline bool oa_clustered(double sij, double r, double li, double lj, double nu) {
+    const double c = fmax(fabs(sij + lj * r), fabs(sij + li * r)) + nu;
+    return !(c <= kPairTau * fabs(lj - li));
+}
 
 // lam_i = S_ii / G_ii
 __global__ void oa_lambda_kernel(const double *S, const double *G, i64 ld, int n, double *lam) {
@@ -115,12 +135,12 @@ __global__ __launch_bounds__(256) void sym_frob_kernel(const double *A, i64 ld, 
 }
 
 // E (written over G, both triangles) from the lower triangles of S = X^T A X and
-// G = X^T X, lam and delta. The thread of a pair i > j reads S_ij and G_ij and
-// writes E_ij and E_ji, so the update in place is race-free; clustered pairs are
-// counted once. One value s_ij serves both entries: E + E^T then equals R to
+// G = X^T X, lam and the pair test. The thread of a pair i > j reads S_ij and G_ij
+// and writes E_ij and E_ji, so the update in place is race-free; clustered pairs
+// are counted once. One value s_ij serves both entries: E + E^T then equals R to
 // rounding, which the orthogonality's convergence needs.
 __global__ __launch_bounds__(256) void oa_e_kernel(const double *S, double *G, i64 ld, int n,
-                                                   const double *lam, double delta,
+                                                   const double *lam, double nu,
                                                    unsigned *nclustered) {
     const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (i64)n * n) return;
@@ -134,7 +154,7 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double *S, double *G, i
     const double r = -G[o], sij = S[o];
     const double gap = lam[j] - lam[i]; // E_ij divides by lam_j - lam_i, E_ji by its negative
     double eij, eji;
-    if (fabs(gap) > delta) {
+    if (!oa_clustered(sij, r, lam[i], lam[j], nu)) {
         eij = (sij + lam[j] * r) / gap;
         eji = -(sij + lam[i] * r) / gap;
     } else {
@@ -143,6 +163,21 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double *S, double *G, i
     }
     G[o] = eij;
     G[j + (i64)i * ld] = eji;
+}
+
+// flag[i] = flag[j] = 1 for every clustered pair (i, j) (flag cleared beforehand;
+// every writer stores the same value)
+__global__ __launch_bounds__(256) void oa_flag_kernel(const double *S, const double *G, i64 ld, int n,
+                                                      const double *lam, double nu, int *flag) {
+    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    if (i <= j) return;
+    const i64 o = i + (i64)j * ld;
+    if (oa_clustered(S[o], -G[o], lam[i], lam[j], nu)) {
+        flag[i] = 1;
+        flag[j] = 1;
+    }
 }
 
 // dst[:, t] = X[:, idx[t]] for t < m (n rows)
@@ -527,6 +562,8 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     CUEV_HIP(hipStreamSynchronize(s));
     const double afro = std::sqrt(sum_parts(hp, 0));
     const double anorm = afro > 0 ? afro : 1.0;
+    // the rounding level of an entry of S = X^T A X: n u ||A||_F
+    const double nu = (double)n * 0x1p-53 * anorm;
     // X alternates between evec and T1; the other buffer holds A X, then the next iterate
     double *X = evec;
     // sweeps counts every evaluation-and-update (Newton and Rayleigh-Ritz); the
@@ -553,44 +590,38 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         CUEV_LAUNCH_CHECK();
         CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * hp.size(), hipMemcpyDeviceToHost, s));
         CUEV_HIP(hipStreamSynchronize(s));
-        const double s_off = std::sqrt(sum_parts(hp, 0)), r_fro = std::sqrt(sum_parts(hp, 1));
+        const double r_fro = std::sqrt(sum_parts(hp, 1));
         res = std::sqrt(sum_parts(hp, 2)) / ((double)n * anorm);
         orth = r_fro / n;
         if ((res <= tol && orth <= tol) || oa_sweeps >= max_sweeps) break;
-        std::vector<double> hl(n);
-        CUEV_HIP(hipMemcpyAsync(hl.data(), mb.lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
-        CUEV_HIP(hipStreamSynchronize(s));
-        double lmax = 0;
-        for (double v : hl) lmax = std::max(lmax, std::fabs(v));
-        const double delta = 2.0 * (s_off + lmax * r_fro);
 
-        // Clusters (runs of the sorted lam with neighbours closer than delta): the
-        // Newton step only orthogonalises inside a cluster, so the vectors there stay
-        // at the FP32 start's accuracy. Before each Newton step a Rayleigh-Ritz step
-        // on every cluster resolves them (the cluster treatment of Ogita and
-        // Aishima's refinement for clustered eigenvalues, JJIAM 36, 2019); S and G
-        // are then recomputed for the rotated basis.
-        if (!did_rr) {
-            std::vector<int> ord(n);
-            std::iota(ord.begin(), ord.end(), 0);
-            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hl[a] < hl[b]; });
-            std::vector<int> idx, start;
-            for (int i = 0; i < n;) {
-                int j = i + 1;
-                while (j < n && hl[ord[j]] - hl[ord[j - 1]] <= delta) ++j;
-                if (j - i >= 2) {
-                    start.push_back((int)idx.size());
-                    for (int t = i; t < j; ++t) idx.push_back(ord[t]);
-                }
-                i = j;
-            }
+        // After a Newton step, the columns of the clustered pairs: that step only
+        // orthogonalised them against each other, so they stay at the FP32 start's
+        // accuracy within their clusters, and one Rayleigh-Ritz step on their union
+        // resolves them in the basis the step has already refined everywhere else;
+        // S and G are then recomputed for the rotated basis.
+        if (oa_sweeps > 0 && !did_rr) {
+            CUEV_TRY(fill_int(s, mb.perm, n, 0));
+            hipLaunchKernelGGL(oa_flag_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
+                               mb.S, mb.G, (i64)n, n, mb.lam, nu, mb.perm);
+            CUEV_LAUNCH_CHECK();
+            std::vector<double> hl(n);
+            std::vector<int> flag(n);
+            CUEV_HIP(hipMemcpyAsync(hl.data(), mb.lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+            CUEV_HIP(hipMemcpyAsync(flag.data(), mb.perm, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+            CUEV_HIP(hipStreamSynchronize(s));
+            std::vector<int> idx;
+            for (int i = 0; i < n; ++i)
+                if (flag[i]) idx.push_back(i);
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hl[a] < hl[b]; });
             const int m = (int)idx.size();
             if (m > 0) {
-                start.push_back(m);
-                if (m >= kRRDeviceMin)
+                if (m >= kRRDeviceMin) {
                     CUEV_TRY(rayleigh_ritz_gpu(mb, s, X, other, n, idx, gs64));
-                else
+                } else {
+                    const std::vector<int> start{0, m}; // the union as one cluster
                     CUEV_TRY(rayleigh_ritz(mb, s, X, other, n, idx, start, gs64));
+                }
                 nclus_cols = std::max(nclus_cols, m);
                 did_rr = true;
                 ++sweeps;
@@ -601,7 +632,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         // E over G, then X <- X + X E
         CUEV_TRY(fill_int(s, reinterpret_cast<int *>(mb.count), 1, 0));
         hipLaunchKernelGGL(oa_e_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
-                           mb.S, mb.G, (i64)n, n, mb.lam, delta, mb.count);
+                           mb.S, mb.G, (i64)n, n, mb.lam, nu, mb.count);
         CUEV_LAUNCH_CHECK();
         // next iterate X + X E into `other` (A X is no longer needed)
         CUEV_TRY(copy2d<double>(s, X, n, other, n, n, n));

@@ -32,9 +32,9 @@ def test_mixed_random_symmetric(torch_dev, n):
 @pytest.mark.parametrize("n", [500, 2048])
 def test_mixed_geometric_with_cluster(torch_dev, n):
     """Scaled-down mixed16k: geometric spectrum 1..1e-6, random signs, 10% in a 1e-9 cluster.
-    At n=2048 more than 512 columns are clustered, so the Rayleigh-Ritz step runs on the
-    GPU (one FP64 solve of the projected union); at n=500 (fewer than 512 columns in all)
-    it runs per cluster on the host."""
+    At n=2048 more than 512 columns are in clustered pairs, so the Rayleigh-Ritz step runs
+    on the GPU (one FP64 solve of the projected union); at n=500 (fewer than 512 columns)
+    it runs as one host dsygvd on the union."""
     torch = torch_dev
     from cuev_amd import workloads
     Q = workloads.orthogonal_q(torch, n, 1, "cuda", torch.float64)
