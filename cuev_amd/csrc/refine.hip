@@ -73,18 +73,30 @@ __device__ inline bool oa_clustered(double sij, double r, double li, double lj, 
 }
 
 // lam_i = S_ii / G_ii
-__global__ void oa_lambda_kernel(const double *S, const double *G, i64 ld, int n, double *lam) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) lam[i] = S[i + (i64)i * ld] / G[i + (i64)i * ld];
+// with S_ii = x_i . (A X)_i: one workgroup per column, so the convergence check
+// needs no product for S (formed only when the refinement goes on)
+__global__ __launch_bounds__(256) void oa_lambda_kernel(const double *X, const double *AX, const double *G,
+                                                        i64 ld, int n, double *lam) {
+    __shared__ double red[256];
+    const int i = blockIdx.x;
+    const double *x = X + (i64)i * ld, *ax = AX + (i64)i * ld;
+    double a = 0;
+    for (int r = threadIdx.x; r < n; r += 256) a += x[r] * ax[r];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lam[i] = red[0] / G[i + (i64)i * ld];
 }
 
-// block partial sums of squares (S and G symmetric, lower triangles stored):
-//  part[0*kRedBlocks + b] : ||S - diag(lam)||_F^2
+// block partial sums of squares (G symmetric, lower triangle stored):
 //  part[1*kRedBlocks + b] : ||G - I||_F^2            (= ||R||_F^2)
 //  part[2*kRedBlocks + b] : ||AX - X diag(lam)||_F^2
-__global__ __launch_bounds__(256) void oa_norms_kernel(const double *S, const double *G,
-                                                       const double *AX, const double *X, i64 ld,
-                                                       int n, const double *lam, double *part) {
+// (part[0*kRedBlocks + b] is written as zero)
+__global__ __launch_bounds__(256) void oa_norms_kernel(const double *G, const double *AX, const double *X,
+                                                       i64 ld, int n, const double *lam, double *part) {
     __shared__ double red[3][256];
     double a = 0, g = 0, r = 0;
     const i64 total = (i64)n * n;
@@ -96,9 +108,7 @@ __global__ __launch_bounds__(256) void oa_norms_kernel(const double *S, const do
         r += t * t;
         if (i < j) continue;
         const double w = (i == j) ? 1.0 : 2.0;
-        const double sv = S[o] - (i == j ? lam[i] : 0.0);
         const double q = G[o] - (i == j ? 1.0 : 0.0);
-        a += w * sv * sv;
         g += w * q * q;
     }
     red[0][threadIdx.x] = a;
@@ -574,19 +584,19 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     unsigned nclus = 0;
     const auto ref0 = clk::now();
     for (;;) {
-        // A X (2n^3), the lower triangles of S and G (n^3 each), X E (2n^3)
+        // A X (2n^3), the lower triangles of G and S (n^3 each; S only when the
+        // refinement goes on), X E (2n^3)
         prof::Scope sc(s, "mixed.refine_sweep", 6.0 * (double)n * n * n);
         double *other = (X == evec) ? mb.T1 : evec;
-        // AX = A X (A lower) into the buffer not holding X, S = X^T AX, G = X^T X
+        // AX = A X (A lower) into the buffer not holding X, G = X^T X (lower triangle),
+        // lam from diag(X^T AX) and diag(G)
         CUEV_TRY(symm_lower<double>(s, n, n, 1.0, A, n, X, n, 0.0, other, n, gs64));
-        // S and G are symmetric: lower triangles only (the element-wise kernels read them so)
-        CUEV_TRY(gemm_ex<double>(s, n, n, n, 1.0, opnd(X, n, 1), opnd(other, n, 0), 0.0, mb.S, n, 1, gs64));
         CUEV_TRY(gemm_ex<double>(s, n, n, n, 1.0, opnd(X, n, 1), opnd(X, n, 0), 0.0, mb.G, n, 1, gs64));
-        hipLaunchKernelGGL(oa_lambda_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, mb.S,
-                           mb.G, (i64)n, n, mb.lam);
+        hipLaunchKernelGGL(oa_lambda_kernel, dim3((unsigned)n), dim3(256), 0, s, X, other, mb.G, (i64)n, n,
+                           mb.lam);
         CUEV_LAUNCH_CHECK();
-        hipLaunchKernelGGL(oa_norms_kernel, dim3(kRedBlocks), dim3(256), 0, s, mb.S, mb.G, other, X,
-                           (i64)n, n, mb.lam, mb.part);
+        hipLaunchKernelGGL(oa_norms_kernel, dim3(kRedBlocks), dim3(256), 0, s, mb.G, other, X, (i64)n, n,
+                           mb.lam, mb.part);
         CUEV_LAUNCH_CHECK();
         CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * hp.size(), hipMemcpyDeviceToHost, s));
         CUEV_HIP(hipStreamSynchronize(s));
@@ -594,6 +604,8 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         res = std::sqrt(sum_parts(hp, 2)) / ((double)n * anorm);
         orth = r_fro / n;
         if ((res <= tol && orth <= tol) || oa_sweeps >= max_sweeps) break;
+        // S = X^T AX (lower triangle: the element-wise kernels read it so)
+        CUEV_TRY(gemm_ex<double>(s, n, n, n, 1.0, opnd(X, n, 1), opnd(other, n, 0), 0.0, mb.S, n, 1, gs64));
 
         // After a Newton step, the columns of the clustered pairs: that step only
         // orthogonalised them against each other, so they stay at the FP32 start's
