@@ -5,8 +5,10 @@
 // Pipeline (double API, precision = CUEV_PRECISION_MIXED):
 //   1. A32 = (float) A; DBBR and bulge chasing in FP32 (FP32 MFMA, FP32
 //      accumulation; panel norms, dot products, tau and T stay FP64).
-//   2. Host dstedc in FP64 on the FP32-derived (d, e), overlapped with the FP32
-//      build of X32 = Q_s Q_b on the GPU; X = (double)(X32 (float)Q_d).
+//   2. FP64 divide and conquer on the FP32-derived (d, e) (device-resident, or host
+//      dstedc), overlapped with the FP32 build of M = Q_b^T Q_s^T on the GPU (Q_s^T
+//      beside the bulge chasing, as in the FP64 solve's reordered order);
+//      X = (double)(M^T (float)Q_d).
 //   3. Ogita-Aishima sweeps in FP64 (T. Ogita, K. Aishima, "Iterative
 //      refinement for symmetric eigenvalue decomposition", JJIAM 35, 2018,
 //      Algorithm 1):
@@ -508,20 +510,27 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     hipEvent_t *ev = cf.ev;
 
     // 1. FP32 reduction (A32 lives in ws.M until the band is packed)
+    CUEV_TRY(bt_ensure(cf));
+    hipStream_t bs = cf.bt_stream;
     CUEV_HIP(hipEventRecord(ev[0], s));
     CUEV_TRY((convert2d<double, float>(s, A, n, ws.M, ws.ld, n, n)));
     CUEV_TRY(dbbr_reduce<float>(ws, ws.M, ws.B));
     CUEV_HIP(hipEventRecord(ev[1], s));
+    // 2. M = Q_b^T Q_s^T in FP32 on the back-transform stream: Q_s^T beside the
+    // bulge chasing, Q_b^T after it, beside the FP64 D&C
+    CUEV_HIP(hipStreamWaitEvent(bs, ev[1], 0));
+    CUEV_HIP(hipEventRecord(cf.xev[0], bs));
+    CUEV_TRY(sbr_back_t<float>(ws, bs, ws.Y, ws.W, ws.M, ws.ld));
+    CUEV_HIP(hipEventRecord(cf.xev[1], bs));
     CUEV_TRY(bc_chase<float>(ws, ws.B, ws.d, ws.e));
     CUEV_HIP(hipEventRecord(ev[2], s));
     CUEV_HIP(hipMemcpyAsync(cf.h_d, ws.d, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     CUEV_HIP(hipMemcpyAsync(cf.h_e, ws.e, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     CUEV_HIP(hipEventRecord(ev[3], s));
-    // 2. FP32 X32 = Q_s Q_b on the GPU while the host runs the FP64 D&C
-    CUEV_TRY(fill2d<float>(s, ws.M, ws.ld, n, n, 0.f, 1.f));
-    CUEV_TRY(bc_back<float>(ws, ws.U, ws.M, ws.ld));
-    CUEV_TRY(sbr_back<float>(ws, ws.Y, ws.W, ws.M, ws.ld));
-    CUEV_HIP(hipEventRecord(ev[4], s));
+    CUEV_HIP(hipStreamWaitEvent(bs, ev[2], 0));
+    CUEV_HIP(hipEventRecord(cf.xev[2], bs));
+    CUEV_TRY(bc_back_t<float>(ws, bs, ws.U, ws.M, ws.ld));
+    CUEV_HIP(hipEventRecord(cf.xev[3], bs));
     CUEV_HIP(hipEventSynchronize(ev[3]));
     for (int i = 0; i < n; ++i) {
         mb.h_d[i] = cf.h_d[i];
@@ -538,23 +547,28 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     const auto dc0 = clk::now();
     if (dc_mode == CUEV_DC_DEVICE) {
         // Q_d into S and the eigenvalues into lam, on a stream of its own beside the
-        // FP32 back-transform; T1, G and evec (written last) are its work buffers
+        // FP32 back-transform (small tiles, no split-K; the large merges wait for
+        // the back-transform); T1, G and evec (written last) are its work buffers
+        Scratch dcs;
+        dcs.small_tiles = 1;
         CUEV_TRY(dc_solve<double>(*mb.dc, mb.dc_stream, n, mb.h_d, mb.h_e, mb.S, n, mb.lam, mb.T1, mb.G,
-                                  evec, Scratch{}, opt.dc_leaf));
+                                  evec, dcs, opt.dc_leaf, cf.xev[3], kDcGateFrom(n)));
     } else {
         CUEV_TRY(stedc<double>(n, mb.h_d, mb.h_e, mb.h_z, n, mb.h_work, mb.lwork, mb.h_iwork, mb.liwork));
     }
     const auto dc1 = clk::now();
-    CUEV_HIP(hipEventRecord(ev[5], s));
-    // X = (double)(X32 * (float)Q_d): Q_d through S (FP64) into W (FP32, free after sbr_back)
+    // X = (double)(M^T (float)Q_d): Q_d through S (FP64) into W (FP32), and the
+    // product into Y, both free once the back-transform stream is done
     if (dc_mode != CUEV_DC_DEVICE) {
         CUEV_HIP(hipMemcpyAsync(mb.S, mb.h_z, 8 * (size_t)n * n, hipMemcpyHostToDevice, s));
         CUEV_HIP(hipMemcpyAsync(mb.lam, mb.h_d, 8 * (size_t)n, hipMemcpyHostToDevice, s));
     }
+    CUEV_HIP(hipStreamWaitEvent(s, cf.xev[3], 0));
+    CUEV_HIP(hipEventRecord(ev[4], s));
     CUEV_TRY((convert2d<double, float>(s, mb.S, n, ws.W, ws.ld, n, n)));
     {
         prof::Scope sc(s, "mixed.bt_gemm32", 2.0 * (double)n * n * n);
-        CUEV_TRY(gemm<float>(s, 0, 0, n, n, n, 1.f, ws.M, ws.ld, ws.W, ws.ld, 0.f, ws.Y, ws.ld, gs32));
+        CUEV_TRY(gemm<float>(s, 1, 0, n, n, n, 1.f, ws.M, ws.ld, ws.W, ws.ld, 0.f, ws.Y, ws.ld, gs32));
     }
     CUEV_TRY((convert2d<float, double>(s, ws.Y, ws.ld, evec, n, n, n)));
     CUEV_HIP(hipEventRecord(ev[6], s));
@@ -692,8 +706,15 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         timer->dbbr_ms = em(0, 1);
         timer->bc_ms = em(1, 2);
         timer->dc_ms = (float)ms_between(dc0, dc1);
-        timer->overlap_ms = em(3, 4);
-        timer->bt_copyout_ms = em(5, 6);
+        auto emx = [&](int a, int b) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, cf.xev[a], cf.xev[b]);
+            return ms;
+        };
+        timer->bt_qs_ms = emx(0, 1);
+        timer->bt_qb_ms = emx(2, 3);
+        timer->overlap_ms = timer->bt_qs_ms + timer->bt_qb_ms;
+        timer->bt_copyout_ms = em(4, 6);
         timer->bt_ms = (float)ms_between(dc1, ref0);
         timer->refine_ms = (float)ms_between(ref0, ref1);
         timer->total_ms = (float)ms_between(wall0, wall1);

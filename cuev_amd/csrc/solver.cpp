@@ -50,6 +50,8 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
     return ms;
 }
+} // namespace
+
 // Smallest merge of the device D&C that waits for the back-transform running beside
 // it: the three top tree levels at n = 16384, which hold nearly all of the D&C's GEMM
 // work (measured there against other levels, no gate, and no overlap at all), and
@@ -58,7 +60,6 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 // n = 16384). At n = 49152 one solve took 25.9 s with this gate and 27.8 s with the
 // top three levels only, but bc_back alone varies there from 7.6 to 12 s between runs.
 int kDcGateFrom(int n) { return std::min(n / 4, 4096); }
-} // namespace
 
 template <typename T>
 int tridi_dc(Context<T> &ctx, T *d, T *e, T *eval, T *evec, i64 ldv, double *dc_ms) {

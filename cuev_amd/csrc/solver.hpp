@@ -7,6 +7,10 @@ namespace cuev {
 
 void options_defaults(cuev_options &o);
 
+// Smallest merge (rows) of the device D&C that waits for a back-transform running
+// beside it (the gate of dc_solve).
+int kDcGateFrom(int n);
+
 // D&C on device (d, e): eval <- eigenvalues, evec (ld ldv) <- Q_d; host ?stedc
 // or the device-resident form, as ctx.dc_mode says.
 template <typename T>
