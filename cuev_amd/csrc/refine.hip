@@ -222,13 +222,14 @@ __global__ void permute_cols_kernel(const double *X, double *Xp, i64 ld, int n, 
     Xp[i + (i64)j * ld] = X[i + (i64)perm[j] * ld];
 }
 
-// Z = I - E/2 + 3/8 E^2 (the series of (I + E)^{-1/2} to second order) over the
-// m x m block, in place over E; E2 holds E^2.
-__global__ void oa_invsqrt_kernel(double *E, const double *E2, i64 m) {
+// C = S - (ES + (ES)^T) / 2 over the lower triangle of the m x m block, in place
+// over S (the upper triangle is left as it is: the solver reads the lower one)
+__global__ void rr_congruence_kernel(double *S, const double *ES, i64 m) {
     const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m * m) return;
-    const bool diag = (k % m) == (k / m);
-    E[k] = (diag ? 1.0 : 0.0) - 0.5 * E[k] + 0.375 * E2[k];
+    const i64 i = k % m, j = k / m;
+    if (i < j) return;
+    S[k] -= 0.5 * (ES[k] + ES[j + i * m]);
 }
 
 // G - I on the diagonal of the m x m block (in place)
@@ -445,33 +446,32 @@ int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, double *X, double *out, int n
 // Rayleigh-Ritz on the span of all clustered columns at once, on the GPU, for
 // cluster sets too large for the host (the geometric spectra of the mixed
 // benchmark put thousands of eigenvalues within the FP32 start's error of each
-// other). With G_JJ = I + E (||E|| ~ the FP32 start's orthogonality error),
-// Z = I - E/2 + 3/8 E^2 = G_JJ^{-1/2} + O(||E||^3) makes X_J Z orthonormal to
-// that order; C = Z S_JJ Z is solved by the FP64 two-stage solver of this
-// library, C = W~ Theta W~^T, and X_J <- X_J (Z W~). A Rayleigh-Ritz step on a
-// union of clusters is one on a larger subspace, so it resolves every cluster
-// in it. Buffers: `out`, mb.S and mb.G (n x n each) as scratch, mb.lam for Theta.
+// other). It follows a Newton step, so G_JJ = I + E with E at that step's
+// orthogonality error (~1e-9): Z = I - E/2 makes X_J Z orthonormal up to 3/4 E^2,
+// which the next Newton step removes with the rest. C = Z S_JJ Z is formed to the
+// same order as S_JJ - (E S_JJ + (E S_JJ)^T)/2, solved by the FP64 two-stage solver
+// of this library, C = W~ Theta W~^T, and X_J <- X_J (Z W~) with
+// Z W~ = W~ - E W~ / 2. A Rayleigh-Ritz step on a union of clusters is one on a
+// larger subspace, so it resolves every cluster in it. Buffers: `out`, mb.S and
+// mb.G (n x n each) as scratch, mb.lam for Theta.
 int rayleigh_ritz_gpu(MixedBuffers &mb, hipStream_t s, double *X, double *out, int n,
                       const std::vector<int> &idx, Scratch gs) {
     const int m = (int)idx.size();
     const i64 mm = (i64)m * m;
     CUEV_HIP(hipMemcpyAsync(mb.perm, idx.data(), 4 * (size_t)m, hipMemcpyHostToDevice, s));
     const unsigned gsub = (unsigned)ceil_div(mm, 256);
-    double *Sjj = out, *Zb = mb.S, *Tb = mb.G;
+    double *Sjj = out, *Eb = mb.S, *Tb = mb.G;
     // S_JJ into out; then G_JJ into mb.S (S itself is no longer needed)
     hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.S, Sjj, (i64)n, mb.perm, m);
     CUEV_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.G, Zb, (i64)n, mb.perm, m);
+    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.G, Eb, (i64)n, mb.perm, m);
     CUEV_LAUNCH_CHECK();
-    // E = G_JJ - I in place, E^2 into mb.G, then Z over E
-    hipLaunchKernelGGL(sub_identity_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, Zb, (i64)m);
+    // E = G_JJ - I in place; E S_JJ into mb.G; C over S_JJ
+    hipLaunchKernelGGL(sub_identity_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, Eb, (i64)m);
     CUEV_LAUNCH_CHECK();
-    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, Zb, m, Zb, m, 0.0, Tb, m, gs));
-    hipLaunchKernelGGL(oa_invsqrt_kernel, dim3(gsub), dim3(256), 0, s, Zb, Tb, (i64)m);
+    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, Eb, m, Sjj, m, 0.0, Tb, m, gs));
+    hipLaunchKernelGGL(rr_congruence_kernel, dim3(gsub), dim3(256), 0, s, Sjj, Tb, (i64)m);
     CUEV_LAUNCH_CHECK();
-    // C = Z S_JJ Z into out (through T = Z S_JJ in mb.G)
-    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, Zb, m, Sjj, m, 0.0, Tb, m, gs));
-    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, Tb, m, Zb, m, 0.0, Sjj, m, gs));
     // C = W~ Theta W~^T: W~ into mb.G, Theta into mb.lam (C is consumed)
     {
         int b = 0, k = 0;
@@ -488,8 +488,9 @@ int rayleigh_ritz_gpu(MixedBuffers &mb, hipStream_t s, double *X, double *out, i
             return st;
         }
     }
-    // W = Z W~ into out; X_J into mb.S; W into mb.G; X_J W into out; scatter into X
-    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, Zb, m, Tb, m, 0.0, Sjj, m, gs));
+    // W = W~ - E W~ / 2 into out; X_J into mb.S; W into mb.G; X_J W into out; scatter into X
+    CUEV_TRY(copy2d<double>(s, Tb, m, Sjj, m, m, m));
+    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, -0.5, Eb, m, Tb, m, 1.0, Sjj, m, gs));
     const unsigned gcol = (unsigned)ceil_div((i64)n * m, 256);
     hipLaunchKernelGGL(gather_cols_kernel, dim3(gcol), dim3(256), 0, s, X, mb.S, (i64)n, n, mb.perm, m);
     CUEV_LAUNCH_CHECK();
