@@ -125,6 +125,28 @@ __global__ __launch_bounds__(256) void oa_norms_kernel(const double *G, const do
     if (threadIdx.x < 3) part[threadIdx.x * kRedBlocks + blockIdx.x] = red[threadIdx.x][0];
 }
 
+// The strict upper triangle of A from its lower one (64 x 64 tiles through LDS):
+// block (ti, tj), tj <= ti, reads lower tile (ti, tj) and writes the upper tile
+// (tj, ti); a diagonal tile writes only its entries above the diagonal. A is then
+// stored in full, so A X runs as a plain product.
+__global__ __launch_bounds__(256) void symmetrize_lower_kernel(double *A, i64 ld, int n) {
+    __shared__ double t[64][65];
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (tj > ti) return;
+    const int r0 = ti * 64, c0 = tj * 64;
+    const int lr = threadIdx.x & 63, lc = threadIdx.x >> 6;
+    for (int c = lc; c < 64; c += 4) {
+        const int r = r0 + lr, col = c0 + c;
+        if (r < n && col < n) t[lr][c] = A[r + (i64)col * ld];
+    }
+    __syncthreads();
+    // upper element (row, col) = A(col, row) = t[col - r0][row - c0]
+    for (int c = lc; c < 64; c += 4) {
+        const int row = c0 + lr, col = r0 + c;
+        if (row < n && col < n && col > row) A[row + (i64)col * ld] = t[c][lr];
+    }
+}
+
 // ||A||_F^2 of the symmetric matrix stored in the lower triangle
 __global__ __launch_bounds__(256) void sym_frob_kernel(const double *A, i64 ld, int n, double *part) {
     __shared__ double red[256];
@@ -586,6 +608,12 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * (size_t)kRedBlocks, hipMemcpyDeviceToHost, s));
     CUEV_HIP(hipStreamSynchronize(s));
     const double afro = std::sqrt(sum_parts(hp, 0));
+    // A in full (the solve overwrites A; the refinement reads it only as a whole)
+    {
+        const unsigned nt = (unsigned)ceil_div(n, 64);
+        hipLaunchKernelGGL(symmetrize_lower_kernel, dim3(nt, nt), dim3(256), 0, s, A, (i64)n, n);
+        CUEV_LAUNCH_CHECK();
+    }
     const double anorm = afro > 0 ? afro : 1.0;
     // the rounding level of an entry of S = X^T A X: n u ||A||_F
     const double nu = (double)n * 0x1p-53 * anorm;
@@ -603,9 +631,9 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         // refinement goes on), X E (2n^3)
         prof::Scope sc(s, "mixed.refine_sweep", 6.0 * (double)n * n * n);
         double *other = (X == evec) ? mb.T1 : evec;
-        // AX = A X (A lower) into the buffer not holding X, G = X^T X (lower triangle),
+        // AX = A X into the buffer not holding X, G = X^T X (lower triangle),
         // lam from diag(X^T AX) and diag(G)
-        CUEV_TRY(symm_lower<double>(s, n, n, 1.0, A, n, X, n, 0.0, other, n, gs64));
+        CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, A, n, X, n, 0.0, other, n, gs64));
         CUEV_TRY(gemm_ex<double>(s, n, n, n, 1.0, opnd(X, n, 1), opnd(X, n, 0), 0.0, mb.G, n, 1, gs64));
         hipLaunchKernelGGL(oa_lambda_kernel, dim3((unsigned)n), dim3(256), 0, s, X, other, mb.G, (i64)n, n,
                            mb.lam);

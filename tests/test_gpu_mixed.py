@@ -55,3 +55,21 @@ def test_mixed_rejected_for_float(torch_dev):
     with pytest.raises(cv.CuevError) as e:
         cv.symm_eig_solve(A, precision="mixed")
     assert e.value.status == 8
+
+
+@pytest.mark.parametrize("n", [700, 1100])
+def test_mixed_reads_only_lower(torch_dev, n):
+    """Only the lower triangle of A is read: with NaN above the diagonal the mixed form
+    (whose refinement fills A's upper triangle from the lower one before its A X
+    products) still reaches the FP64 tolerances."""
+    rng = np.random.default_rng(n + 7)
+    G = rng.standard_normal((n, n))
+    A = (G + G.T) / 2
+    A_low = np.tril(A) + np.triu(np.full((n, n), np.nan), 1)
+    # the torch API reads A[i, j] with i >= j of the tensor as given
+    lam, V, t = cv.symm_eig_solve(torch_dev.from_numpy(A_low).cuda(), precision="mixed", timer=True)
+    torch_dev.cuda.synchronize()
+    lam, V = lam.cpu().numpy(), V.cpu().numpy()
+    lam_ref = sl.eigh(A, eigvals_only=True, driver="evd")
+    ev, res, orth = eig_metrics(A, lam, V, lam_ref)
+    assert ev < 1e-12 and res < 1e-13 and orth < 1e-13, (ev, res, orth, t)
