@@ -147,6 +147,52 @@ __global__ __launch_bounds__(256) void symmetrize_lower_kernel(double *A, i64 ld
     }
 }
 
+// Gaussian probes for the norm check: R (n x kProbes, ld n) with standard normal
+// entries by the Box-Muller transform of two uniforms taken from the splitmix64
+// finalizer (Steele, Lea and Flood, OOPSLA 2014) of the entry index, and
+// LR = diag(lam) R.
+constexpr int kProbes = 16;
+__global__ void probe_kernel(double *R, double *LR, const double *lam, int n) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (i64)n * kProbes) return;
+    unsigned long long x = (unsigned long long)k + 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    const double u1 = ((double)(x >> 32) + 1.0) * 0x1p-32;       // (0, 1]
+    const double u2 = (double)(x & 0xffffffffull) * 0x1p-32;      // [0, 1)
+    const double r = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    R[k] = r;
+    LR[k] = lam[k % n] * r;
+}
+
+// block partial sums over n x kProbes entries:
+//  part[0*kRedBlocks + b] : ||A X R - X diag(lam) R||_F^2
+//  part[1*kRedBlocks + b] : ||X^T X R - R||_F^2
+__global__ __launch_bounds__(256) void probe_norms_kernel(const double *AY, const double *XLR, const double *XtY,
+                                                          const double *R, i64 total, double *part) {
+    __shared__ double red[2][256];
+    double a = 0, g = 0;
+    for (i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (i64)gridDim.x * blockDim.x) {
+        const double t = AY[idx] - XLR[idx];
+        const double q = XtY[idx] - R[idx];
+        a += t * t;
+        g += q * q;
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = g;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) part[threadIdx.x * kRedBlocks + blockIdx.x] = red[threadIdx.x][0];
+}
+
 // ||A||_F^2 of the symmetric matrix stored in the lower triangle
 __global__ __launch_bounds__(256) void sym_frob_kernel(const double *A, i64 ld, int n, double *part) {
     __shared__ double red[256];
@@ -622,15 +668,47 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     // sweeps counts every evaluation-and-update (Newton and Rayleigh-Ritz); the
     // limit applies to the Newton steps
     int sweeps = 0, oa_sweeps = 0, nclus_cols = 0;
-    bool did_rr = false;
+    bool did_rr = false, probe = false;
     double res = 0, orth = 0;
     unsigned nclus = 0;
     const auto ref0 = clk::now();
     for (;;) {
+        double *other = (X == evec) ? mb.T1 : evec;
+        // After a Newton step, first a randomized check of both norms: with r of
+        // independent standard normal entries, E ||M r||^2 = ||M||_F^2, and the sum
+        // over kProbes probes is at worst (M of rank one) ||M||_F^2 times a chi-square
+        // with kProbes degrees of freedom, so an estimate a factor 4 below the truth
+        // has a probability below 1e-7 at 16 probes. The eigenvalue estimates are those of
+        // the last evaluation: the step changes them only in second order. Both
+        // estimates below a quarter of the tolerance end the refinement without the
+        // full evaluation; otherwise it follows. S and G are free here (6 n kProbes
+        // doubles of S are used).
+        if (probe && n >= 256) {
+            prof::Scope sc(s, "mixed.refine_check", 8.0 * (double)n * n * kProbes);
+            const i64 nk = (i64)n * kProbes;
+            double *R = mb.S, *LR = R + nk, *Y = LR + nk, *AY = Y + nk, *XLR = AY + nk, *XtY = XLR + nk;
+            hipLaunchKernelGGL(probe_kernel, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, s, R, LR, mb.lam, n);
+            CUEV_LAUNCH_CHECK();
+            CUEV_TRY(gemm<double>(s, 0, 0, n, kProbes, n, 1.0, X, n, R, n, 0.0, Y, n, gs64));
+            CUEV_TRY(gemm<double>(s, 0, 0, n, kProbes, n, 1.0, A, n, Y, n, 0.0, AY, n, gs64));
+            CUEV_TRY(gemm<double>(s, 0, 0, n, kProbes, n, 1.0, X, n, LR, n, 0.0, XLR, n, gs64));
+            CUEV_TRY(gemm<double>(s, 1, 0, n, kProbes, n, 1.0, X, n, Y, n, 0.0, XtY, n, gs64));
+            hipLaunchKernelGGL(probe_norms_kernel, dim3(kRedBlocks), dim3(256), 0, s, AY, XLR, XtY, R, nk, mb.part);
+            CUEV_LAUNCH_CHECK();
+            CUEV_HIP(hipMemcpyAsync(hp.data(), mb.part, 8 * 2 * (size_t)kRedBlocks, hipMemcpyDeviceToHost, s));
+            CUEV_HIP(hipStreamSynchronize(s));
+            const double est_res = std::sqrt(sum_parts(hp, 0) / kProbes) / ((double)n * anorm);
+            const double est_orth = std::sqrt(sum_parts(hp, 1) / kProbes) / n;
+            if (est_res <= 0.25 * tol && est_orth <= 0.25 * tol) {
+                res = est_res;
+                orth = est_orth;
+                break;
+            }
+        }
+        probe = false;
         // A X (2n^3), the lower triangles of G and S (n^3 each; S only when the
         // refinement goes on), X E (2n^3)
         prof::Scope sc(s, "mixed.refine_sweep", 6.0 * (double)n * n * n);
-        double *other = (X == evec) ? mb.T1 : evec;
         // AX = A X into the buffer not holding X, G = X^T X (lower triangle),
         // lam from diag(X^T AX) and diag(G)
         CUEV_TRY(gemm<double>(s, 0, 0, n, n, n, 1.0, A, n, X, n, 0.0, other, n, gs64));
@@ -696,6 +774,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         X = other;
         ++sweeps;
         ++oa_sweeps;
+        probe = true;
     }
     const auto ref1 = clk::now();
     // eigenvalues ascending; columns follow

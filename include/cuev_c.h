@@ -85,8 +85,9 @@ typedef struct cuev_solve_timer {
     float overlap_ms;    /* GPU back-transform time hidden under the host D&C */
     int refine_sweeps;   /* mixed form: sweeps performed */
     int n_cluster_cols;  /* mixed form: eigenvector columns treated as clustered */
-    float refine_resid;  /* mixed form: last measured relative residual (FP64) */
-    float refine_orth;   /* mixed form: last measured orthogonality error (FP64) */
+    float refine_resid;  /* mixed form: last measured relative residual (FP64; an estimate
+                            from 16 random probes when that check ended the refinement) */
+    float refine_orth;   /* mixed form: last measured orthogonality error (likewise) */
 } cuev_solve_timer;
 
 /* ------------------------------------------------------------------------ */

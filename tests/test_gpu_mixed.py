@@ -47,6 +47,10 @@ def test_mixed_geometric_with_cluster(torch_dev, n):
     ev, res, orth = eig_metrics(An, lam_n, Vn, lam_x.cpu().numpy())
     assert ev < 1e-12 and res < 1e-13 and orth < 1e-13, (ev, res, orth, t)
     assert (t["n_cluster_cols"] >= 512) == (n == 2048), t
+    # the refinement's last check (here the 16-probe estimate, since n >= 256) agrees
+    # with the residual and orthogonality measured here
+    print(f"n={n}: refine_resid/res {t['refine_resid'] / res:.3f} refine_orth/orth {t['refine_orth'] / orth:.3f}")
+    assert 0.1 < t["refine_resid"] / res < 10 and 0.1 < t["refine_orth"] / orth < 10, (res, orth, t)
 
 
 def test_mixed_rejected_for_float(torch_dev):
