@@ -274,14 +274,6 @@ __global__ void scatter_cols_kernel(const double *src, double *X, i64 ld, int n,
     const int i = (int)(k % n), t = (int)(k / n);
     X[i + (i64)idx[t] * ld] = src[i + (i64)t * n];
 }
-// Sj[a + b*m] = S[idx[a], idx[b]] for symmetric S stored in its lower triangle
-__global__ void gather_sub_kernel(const double *S, double *Sj, i64 ld, const int *idx, int m) {
-    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (i64)m * m) return;
-    const int a = idx[k % m], b = idx[k / m];
-    Sj[k] = S[max(a, b) + (i64)min(a, b) * ld];
-}
-
 // Xp[:, j] = X[:, perm[j]]
 __global__ void permute_cols_kernel(const double *X, double *Xp, i64 ld, int n, const int *perm) {
     const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,120 +443,90 @@ double sum_parts(const std::vector<double> &h, int which) {
     return s;
 }
 
-// Rayleigh-Ritz on each cluster (columns idx, clusters [start[c], start[c+1])):
-// solve S_JJ w = theta G_JJ w (host dsygvd, so the rotated columns also come out
-// G-orthonormal within the cluster) and replace X_J by X_J W. S (= X^T A X) and
-// G (= X^T X) are overwritten; `out` is an n x n buffer that holds nothing live.
-int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, double *X, double *out, int n,
-                  const std::vector<int> &idx, const std::vector<int> &start, Scratch gs) {
-    const int m = (int)idx.size();
-    const LapackFns *L = lapack();
-    if (!L || !L->dsygvd)
-        CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE, "mixed: host dsygvd needed for clustered eigenvalues");
-    CUEV_HIP(hipMemcpyAsync(mb.perm, idx.data(), 4 * (size_t)m, hipMemcpyHostToDevice, s));
-    // S_JJ, then G_JJ, as m x m matrices through `out` to the host
-    const unsigned gsub = (unsigned)ceil_div((i64)m * m, 256);
-    double *Sh = mb.h_z; // pinned, n*n >= m*m doubles
-    std::vector<double> gvec((size_t)m * m);
-    const double *Gh = gvec.data();
-    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.S, out, (i64)n, mb.perm, m);
-    CUEV_LAUNCH_CHECK();
-    CUEV_HIP(hipMemcpyAsync(Sh, out, 8 * (size_t)m * m, hipMemcpyDeviceToHost, s));
-    CUEV_HIP(hipStreamSynchronize(s));
-    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.G, out, (i64)n, mb.perm, m);
-    CUEV_LAUNCH_CHECK();
-    CUEV_HIP(hipMemcpyAsync(gvec.data(), out, 8 * (size_t)m * m, hipMemcpyDeviceToHost, s));
-    CUEV_HIP(hipStreamSynchronize(s));
-    std::vector<double> W((size_t)m * m, 0.0), sb, gb, wv, work;
-    std::vector<int> iwork;
-    for (size_t c = 0; c + 1 < start.size(); ++c) {
-        const int a = start[c], sz = start[c + 1] - start[c];
-        sb.assign((size_t)sz * sz, 0.0);
-        gb.assign((size_t)sz * sz, 0.0);
-        for (int q = 0; q < sz; ++q)
-            for (int p = 0; p < sz; ++p) {
-                sb[p + (size_t)q * sz] = Sh[(a + p) + (size_t)(a + q) * m];
-                gb[p + (size_t)q * sz] = Gh[(a + p) + (size_t)(a + q) * m];
-            }
-        wv.assign(sz, 0.0);
-        const int itype = 1;
-        const char jobz = 'V', uplo = 'L';
-        int nn = sz, lda = sz, info = 0;
-        int lwork = 1 + 6 * sz + 2 * sz * sz, liwork = 3 + 5 * sz;
-        work.resize(lwork);
-        iwork.resize(liwork);
-        L->dsygvd(&itype, &jobz, &uplo, &nn, sb.data(), &lda, gb.data(), &lda, wv.data(), work.data(),
-                  &lwork, iwork.data(), &liwork, &info);
-        if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "mixed: dsygvd on a cluster returned %d", info);
-        for (int q = 0; q < sz; ++q)
-            for (int p = 0; p < sz; ++p) W[(a + p) + (size_t)(a + q) * m] = sb[p + (size_t)q * sz];
-    }
-    // X_J into S (n x m), W into G (m x m), X_J W into out, scattered back into X
-    CUEV_HIP(hipMemcpyAsync(mb.G, W.data(), 8 * (size_t)m * m, hipMemcpyHostToDevice, s));
-    const unsigned gcol = (unsigned)ceil_div((i64)n * m, 256);
-    hipLaunchKernelGGL(gather_cols_kernel, dim3(gcol), dim3(256), 0, s, X, mb.S, (i64)n, n, mb.perm, m);
-    CUEV_LAUNCH_CHECK();
-    CUEV_TRY(gemm<double>(s, 0, 0, n, m, m, 1.0, mb.S, n, mb.G, m, 0.0, out, n, gs));
-    hipLaunchKernelGGL(scatter_cols_kernel, dim3(gcol), dim3(256), 0, s, out, X, (i64)n, n, mb.perm, m);
-    CUEV_LAUNCH_CHECK();
-    CUEV_HIP(hipStreamSynchronize(s));
-    return CUEV_STATUS_SUCCESS;
-}
-
-// Rayleigh-Ritz on the span of all clustered columns at once, on the GPU, for
-// cluster sets too large for the host (the geometric spectra of the mixed
-// benchmark put thousands of eigenvalues within the FP32 start's error of each
-// other). It follows a Newton step, so G_JJ = I + E with E at that step's
-// orthogonality error (~1e-9): Z = I - E/2 makes X_J Z orthonormal up to 3/4 E^2,
-// which the next Newton step removes with the rest. C = Z S_JJ Z is formed to the
-// same order as S_JJ - (E S_JJ + (E S_JJ)^T)/2, solved by the FP64 two-stage solver
-// of this library, C = W~ Theta W~^T, and X_J <- X_J (Z W~) with
-// Z W~ = W~ - E W~ / 2. A Rayleigh-Ritz step on a union of clusters is one on a
-// larger subspace, so it resolves every cluster in it. Buffers: `out`, mb.S and
-// mb.G (n x n each) as scratch, mb.lam for Theta.
-int rayleigh_ritz_gpu(MixedBuffers &mb, hipStream_t s, double *X, double *out, int n,
-                      const std::vector<int> &idx, Scratch gs) {
+// Rayleigh-Ritz on the span of the columns idx of X, all at once (a Rayleigh-Ritz
+// step on a union of clusters is one on a larger subspace, so it resolves every
+// cluster in it). From X_J (gathered into mb.S), A X_J (into F) gives the lower
+// triangles of S_JJ = X_J^T A X_J (mb.G) and G_JJ = X_J^T X_J (F, over A X_J).
+//  * Fewer than kRRDeviceMin columns: host dsygvd solves S_JJ w = theta G_JJ w, so
+//    the rotated columns also come out G-orthonormal, and X_J <- X_J W.
+//  * Else on the GPU: the step follows a Newton step, so G_JJ = I + E with E at that
+//    step's orthogonality error (~1e-9): Z = I - E/2 makes X_J Z orthonormal up to
+//    3/4 E^2, which the next Newton step removes. C = Z S_JJ Z is formed to the same
+//    order as S_JJ - (E S_JJ + (E S_JJ)^T)/2 and solved by the FP64 two-stage solver
+//    of this library, C = W~ Theta W~^T, and X_J <- X_J (Z W~) with
+//    Z W~ = W~ - E W~ / 2.
+// A is stored in full; F and mb.S, mb.G (n x n each) are scratch, mb.lam takes Theta.
+int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, const double *A, double *X, double *F, int n,
+                  const std::vector<int> &idx, Scratch gs) {
     const int m = (int)idx.size();
     const i64 mm = (i64)m * m;
     CUEV_HIP(hipMemcpyAsync(mb.perm, idx.data(), 4 * (size_t)m, hipMemcpyHostToDevice, s));
-    const unsigned gsub = (unsigned)ceil_div(mm, 256);
-    double *Sjj = out, *Eb = mb.S, *Tb = mb.G;
-    // S_JJ into out; then G_JJ into mb.S (S itself is no longer needed)
-    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.S, Sjj, (i64)n, mb.perm, m);
-    CUEV_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gather_sub_kernel, dim3(gsub), dim3(256), 0, s, mb.G, Eb, (i64)n, mb.perm, m);
-    CUEV_LAUNCH_CHECK();
-    // E = G_JJ - I in place; E S_JJ into mb.G; C over S_JJ
-    hipLaunchKernelGGL(sub_identity_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, Eb, (i64)m);
-    CUEV_LAUNCH_CHECK();
-    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, Eb, m, Sjj, m, 0.0, Tb, m, gs));
-    hipLaunchKernelGGL(rr_congruence_kernel, dim3(gsub), dim3(256), 0, s, Sjj, Tb, (i64)m);
-    CUEV_LAUNCH_CHECK();
-    // C = W~ Theta W~^T: W~ into mb.G, Theta into mb.lam (C is consumed)
-    {
-        int b = 0, k = 0;
-        CUEV_TRY(resolve_blocking(m, b, k));
-        if (k > m) k = (int)round_up(m, b);
-        Context<double> *ctx = nullptr;
-        CUEV_TRY(context_acquire<double>(ctx, m, b, k, s));
-        const int st = solve_native<double>(*ctx, Sjj, mb.lam, Tb, nullptr, CUEV_BT_AUTO, CUEV_DC_AUTO, 0);
-        if (st == CUEV_STATUS_SUCCESS) {
-            context_release<double>(ctx);
-        } else {
-            (void)hipStreamSynchronize(s);
-            context_destroy<double>(ctx);
-            return st;
-        }
-    }
-    // W = W~ - E W~ / 2 into out; X_J into mb.S; W into mb.G; X_J W into out; scatter into X
-    CUEV_TRY(copy2d<double>(s, Tb, m, Sjj, m, m, m));
-    CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, -0.5, Eb, m, Tb, m, 1.0, Sjj, m, gs));
     const unsigned gcol = (unsigned)ceil_div((i64)n * m, 256);
-    hipLaunchKernelGGL(gather_cols_kernel, dim3(gcol), dim3(256), 0, s, X, mb.S, (i64)n, n, mb.perm, m);
+    double *XJ = mb.S, *Sjj = mb.G, *Gjj = F;
+    hipLaunchKernelGGL(gather_cols_kernel, dim3(gcol), dim3(256), 0, s, X, XJ, (i64)n, n, mb.perm, m);
     CUEV_LAUNCH_CHECK();
-    CUEV_TRY(copy2d<double>(s, Sjj, m, mb.G, m, m, m));
-    CUEV_TRY(gemm<double>(s, 0, 0, n, m, m, 1.0, mb.S, n, mb.G, m, 0.0, out, n, gs));
-    hipLaunchKernelGGL(scatter_cols_kernel, dim3(gcol), dim3(256), 0, s, out, X, (i64)n, n, mb.perm, m);
+    CUEV_TRY(gemm<double>(s, 0, 0, n, m, n, 1.0, A, n, XJ, n, 0.0, F, n, gs));
+    CUEV_TRY(gemm_ex<double>(s, m, m, n, 1.0, opnd(XJ, n, 1), opnd(static_cast<const double *>(F), n, 0), 0.0,
+                             Sjj, m, 1, gs));
+    CUEV_TRY(gemm_ex<double>(s, m, m, n, 1.0, opnd(XJ, n, 1), opnd(XJ, n, 0), 0.0, Gjj, m, 1, gs));
+    if (m < kRRDeviceMin) {
+        const LapackFns *L = lapack();
+        if (!L || !L->dsygvd)
+            CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE, "mixed: host dsygvd needed for clustered eigenvalues");
+        std::vector<double> sb((size_t)mm), gb((size_t)mm), wv(m);
+        CUEV_HIP(hipMemcpyAsync(sb.data(), Sjj, 8 * (size_t)mm, hipMemcpyDeviceToHost, s));
+        CUEV_HIP(hipMemcpyAsync(gb.data(), Gjj, 8 * (size_t)mm, hipMemcpyDeviceToHost, s));
+        CUEV_HIP(hipStreamSynchronize(s));
+        const int itype = 1;
+        const char jobz = 'V', uplo = 'L';
+        int nn = m, lda = m, info = 0;
+        int lwork = 1 + 6 * m + 2 * m * m, liwork = 3 + 5 * m;
+        std::vector<double> work(lwork);
+        std::vector<int> iwork(liwork);
+        L->dsygvd(&itype, &jobz, &uplo, &nn, sb.data(), &lda, gb.data(), &lda, wv.data(), work.data(), &lwork,
+                  iwork.data(), &liwork, &info);
+        if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "mixed: dsygvd on the clustered columns returned %d", info);
+        // W (in sb) into mb.G, X_J W into F, scattered back into X
+        CUEV_HIP(hipMemcpyAsync(mb.G, sb.data(), 8 * (size_t)mm, hipMemcpyHostToDevice, s));
+    } else {
+        const unsigned gsub = (unsigned)ceil_div(mm, 256), mt = (unsigned)ceil_div(m, 64);
+        // S_JJ and E = G_JJ - I in full; E S_JJ into mb.S (X_J is gathered again below);
+        // C over the lower triangle of S_JJ
+        hipLaunchKernelGGL(symmetrize_lower_kernel, dim3(mt, mt), dim3(256), 0, s, Sjj, (i64)m, m);
+        CUEV_LAUNCH_CHECK();
+        hipLaunchKernelGGL(symmetrize_lower_kernel, dim3(mt, mt), dim3(256), 0, s, Gjj, (i64)m, m);
+        CUEV_LAUNCH_CHECK();
+        hipLaunchKernelGGL(sub_identity_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, Gjj, (i64)m);
+        CUEV_LAUNCH_CHECK();
+        const double *E = Gjj;
+        CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, 1.0, E, m, Sjj, m, 0.0, mb.S, m, gs));
+        hipLaunchKernelGGL(rr_congruence_kernel, dim3(gsub), dim3(256), 0, s, Sjj, mb.S, (i64)m);
+        CUEV_LAUNCH_CHECK();
+        // C = W~ Theta W~^T: W~ into mb.S, Theta into mb.lam (C is consumed)
+        {
+            int b = 0, k = 0;
+            CUEV_TRY(resolve_blocking(m, b, k));
+            if (k > m) k = (int)round_up(m, b);
+            Context<double> *ctx = nullptr;
+            CUEV_TRY(context_acquire<double>(ctx, m, b, k, s));
+            const int st = solve_native<double>(*ctx, Sjj, mb.lam, mb.S, nullptr, CUEV_BT_AUTO, CUEV_DC_AUTO, 0);
+            if (st == CUEV_STATUS_SUCCESS) {
+                context_release<double>(ctx);
+            } else {
+                (void)hipStreamSynchronize(s);
+                context_destroy<double>(ctx);
+                return st;
+            }
+        }
+        // W = W~ - E W~ / 2 into mb.G
+        CUEV_TRY(copy2d<double>(s, mb.S, m, mb.G, m, m, m));
+        CUEV_TRY(gemm<double>(s, 0, 0, m, m, m, -0.5, E, m, mb.S, m, 1.0, mb.G, m, gs));
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(gcol), dim3(256), 0, s, X, XJ, (i64)n, n, mb.perm, m);
+        CUEV_LAUNCH_CHECK();
+    }
+    // X_J W into F, scattered back into X
+    CUEV_TRY(gemm<double>(s, 0, 0, n, m, m, 1.0, XJ, n, mb.G, m, 0.0, F, n, gs));
+    hipLaunchKernelGGL(scatter_cols_kernel, dim3(gcol), dim3(256), 0, s, F, X, (i64)n, n, mb.perm, m);
     CUEV_LAUNCH_CHECK();
     CUEV_HIP(hipStreamSynchronize(s));
     return CUEV_STATUS_SUCCESS;
@@ -669,11 +631,20 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     // limit applies to the Newton steps
     int sweeps = 0, oa_sweeps = 0, nclus_cols = 0;
     bool did_rr = false, probe = false;
+    std::vector<int> rr_cols; // columns of the Rayleigh-Ritz step due right after the next Newton step
     double res = 0, orth = 0;
     unsigned nclus = 0;
     const auto ref0 = clk::now();
     for (;;) {
         double *other = (X == evec) ? mb.T1 : evec;
+        if (!rr_cols.empty()) {
+            CUEV_TRY(rayleigh_ritz(mb, s, A, X, other, n, rr_cols, gs64));
+            nclus_cols = std::max(nclus_cols, (int)rr_cols.size());
+            rr_cols.clear();
+            did_rr = true;
+            ++sweeps;
+            continue;
+        }
         // After a Newton step, first a randomized check of both norms: with r of
         // independent standard normal entries, E ||M r||^2 = ||M||_F^2, and the sum
         // over kProbes probes is at worst (M of rank one) ||M||_F^2 times a chi-square
@@ -728,12 +699,15 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         // S = X^T AX (lower triangle: the element-wise kernels read it so)
         CUEV_TRY(gemm_ex<double>(s, n, n, n, 1.0, opnd(X, n, 1), opnd(other, n, 0), 0.0, mb.S, n, 1, gs64));
 
-        // After a Newton step, the columns of the clustered pairs: that step only
-        // orthogonalised them against each other, so they stay at the FP32 start's
-        // accuracy within their clusters, and one Rayleigh-Ritz step on their union
-        // resolves them in the basis the step has already refined everywhere else;
-        // S and G are then recomputed for the rotated basis.
-        if (oa_sweeps > 0 && !did_rr) {
+        // The columns of the clustered pairs. The Newton step only orthogonalises
+        // them against each other, so they stay at the FP32 start's accuracy within
+        // their clusters, and one Rayleigh-Ritz step on their union, after that step,
+        // resolves them in a basis it has refined everywhere else. Before the first
+        // Newton step they are taken from this evaluation (the step leaves the same
+        // pairs clustered), and the Rayleigh-Ritz step follows the step directly;
+        // later, after a Newton step, from the evaluation that follows it. S and G
+        // are recomputed for the rotated basis.
+        if (!did_rr) {
             CUEV_TRY(fill_int(s, mb.perm, n, 0));
             hipLaunchKernelGGL(oa_flag_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
                                mb.S, mb.G, (i64)n, n, mb.lam, nu, mb.perm);
@@ -747,15 +721,11 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
             for (int i = 0; i < n; ++i)
                 if (flag[i]) idx.push_back(i);
             std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hl[a] < hl[b]; });
-            const int m = (int)idx.size();
-            if (m > 0) {
-                if (m >= kRRDeviceMin) {
-                    CUEV_TRY(rayleigh_ritz_gpu(mb, s, X, other, n, idx, gs64));
-                } else {
-                    const std::vector<int> start{0, m}; // the union as one cluster
-                    CUEV_TRY(rayleigh_ritz(mb, s, X, other, n, idx, start, gs64));
-                }
-                nclus_cols = std::max(nclus_cols, m);
+            if (oa_sweeps == 0) {
+                rr_cols = std::move(idx);
+            } else if (!idx.empty()) {
+                CUEV_TRY(rayleigh_ritz(mb, s, A, X, other, n, idx, gs64));
+                nclus_cols = std::max(nclus_cols, (int)idx.size());
                 did_rr = true;
                 ++sweeps;
                 continue;
@@ -774,7 +744,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         X = other;
         ++sweeps;
         ++oa_sweeps;
-        probe = true;
+        probe = rr_cols.empty(); // no check when the Rayleigh-Ritz step follows
     }
     const auto ref1 = clk::now();
     // eigenvalues ascending; columns follow
