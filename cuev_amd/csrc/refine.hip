@@ -245,18 +245,19 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double *S, double *G, i
     G[j + (i64)i * ld] = eji;
 }
 
-// flag[i] = flag[j] = 1 for every clustered pair (i, j) (flag cleared beforehand;
-// every writer stores the same value)
+// For every clustered pair (i, j): the largest rank (position in the ascending order
+// of lam) among each column's clustered partners, hi (set beforehand to -1; hi[i] >= 0
+// marks a clustered column).
 __global__ __launch_bounds__(256) void oa_flag_kernel(const double *S, const double *G, i64 ld, int n,
-                                                      const double *lam, double nu, int *flag) {
+                                                      const double *lam, double nu, const int *rank, int *hi) {
     const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (i64)n * n) return;
     const int i = (int)(idx % n), j = (int)(idx / n);
     if (i <= j) return;
     const i64 o = i + (i64)j * ld;
     if (oa_clustered(S[o], -G[o], lam[i], lam[j], nu)) {
-        flag[i] = 1;
-        flag[j] = 1;
+        atomicMax(hi + i, rank[j]);
+        atomicMax(hi + j, rank[i]);
     }
 }
 
@@ -305,6 +306,7 @@ struct MixedBuffers {
     double *part = nullptr;                           // 3 * kRedBlocks
     unsigned *count = nullptr;
     int *perm = nullptr;                              // n
+    int *rank = nullptr, *hi = nullptr;               // n each (the cluster sets)
     double *h_d = nullptr, *h_e = nullptr, *h_z = nullptr; // pinned
     double *h_work = nullptr;
     int64_t *h_iwork = nullptr;
@@ -334,7 +336,7 @@ int mixed_alloc(MixedBuffers *&out, int n) {
     (void)hipGetDevice(&m->device);
     const size_t nn = (size_t)n * n;
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t bytes = 3 * al(nn * 8) + al(n * 8) + al(3 * kRedBlocks * 8) + al(64) + al(n * 4);
+    const size_t bytes = 3 * al(nn * 8) + al(n * 8) + al(3 * kRedBlocks * 8) + al(64) + 3 * al(n * 4);
     if (hipMalloc(&m->pool, bytes) != hipSuccess) {
         m->pool = nullptr;
         mixed_free(m);
@@ -354,6 +356,10 @@ int mixed_alloc(MixedBuffers *&out, int n) {
     m->count = reinterpret_cast<unsigned *>(p);
     p += al(64);
     m->perm = reinterpret_cast<int *>(p);
+    p += al(n * 4);
+    m->rank = reinterpret_cast<int *>(p);
+    p += al(n * 4);
+    m->hi = reinterpret_cast<int *>(p);
     if (hipHostMalloc(&m->h_d, n * 8, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&m->h_e, n * 8, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&m->h_z, nn * 8, hipHostMallocDefault) != hipSuccess) {
@@ -443,21 +449,23 @@ double sum_parts(const std::vector<double> &h, int which) {
     return s;
 }
 
-// Rayleigh-Ritz on the span of the columns idx of X, all at once (a Rayleigh-Ritz
-// step on a union of clusters is one on a larger subspace, so it resolves every
-// cluster in it). From X_J (gathered into mb.S), A X_J (into F) gives the lower
-// triangles of S_JJ = X_J^T A X_J (mb.G) and G_JJ = X_J^T X_J (F, over A X_J).
-//  * Fewer than kRRDeviceMin columns: host dsygvd solves S_JJ w = theta G_JJ w, so
-//    the rotated columns also come out G-orthonormal, and X_J <- X_J W.
-//  * Else on the GPU: the step follows a Newton step, so G_JJ = I + E with E at that
-//    step's orthogonality error (~1e-9): Z = I - E/2 makes X_J Z orthonormal up to
-//    3/4 E^2, which the next Newton step removes. C = Z S_JJ Z is formed to the same
-//    order as S_JJ - (E S_JJ + (E S_JJ)^T)/2 and solved by the FP64 two-stage solver
-//    of this library, C = W~ Theta W~^T, and X_J <- X_J (Z W~) with
+// Rayleigh-Ritz steps on cluster sets of columns of X: idx holds the sets one after
+// another, set c at [start[c], start[c+1]). From X_J (gathered into mb.S), A X_J (into
+// F) gives the lower triangles of S_JJ = X_J^T A X_J (mb.G) and G_JJ = X_J^T X_J (F,
+// over A X_J); each set's step uses its diagonal blocks. (A Rayleigh-Ritz step on a
+// union of clusters is one on a larger subspace, so it resolves every cluster in it.)
+//  * on_gpu false: host dsygvd solves S_JJ w = theta G_JJ w on each set's block, so
+//    the rotated columns also come out G-orthonormal, and X_J <- X_J W with W block
+//    diagonal.
+//  * on_gpu true (one set): the step follows a Newton step, so G_JJ = I + E with E at
+//    that step's orthogonality error (~1e-9): Z = I - E/2 makes X_J Z orthonormal up
+//    to 3/4 E^2, which the next Newton step removes. C = Z S_JJ Z is formed to the
+//    same order as S_JJ - (E S_JJ + (E S_JJ)^T)/2 and solved by the FP64 two-stage
+//    solver of this library, C = W~ Theta W~^T, and X_J <- X_J (Z W~) with
 //    Z W~ = W~ - E W~ / 2.
 // A is stored in full; F and mb.S, mb.G (n x n each) are scratch, mb.lam takes Theta.
 int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, const double *A, double *X, double *F, int n,
-                  const std::vector<int> &idx, Scratch gs) {
+                  const std::vector<int> &idx, const std::vector<int> &start, bool on_gpu, Scratch gs) {
     const int m = (int)idx.size();
     const i64 mm = (i64)m * m;
     CUEV_HIP(hipMemcpyAsync(mb.perm, idx.data(), 4 * (size_t)m, hipMemcpyHostToDevice, s));
@@ -469,25 +477,39 @@ int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, const double *A, double *X, d
     CUEV_TRY(gemm_ex<double>(s, m, m, n, 1.0, opnd(XJ, n, 1), opnd(static_cast<const double *>(F), n, 0), 0.0,
                              Sjj, m, 1, gs));
     CUEV_TRY(gemm_ex<double>(s, m, m, n, 1.0, opnd(XJ, n, 1), opnd(XJ, n, 0), 0.0, Gjj, m, 1, gs));
-    if (m < kRRDeviceMin) {
+    if (!on_gpu) {
         const LapackFns *L = lapack();
         if (!L || !L->dsygvd)
             CUEV_FAIL(CUEV_STATUS_LAPACK_UNAVAILABLE, "mixed: host dsygvd needed for clustered eigenvalues");
-        std::vector<double> sb((size_t)mm), gb((size_t)mm), wv(m);
-        CUEV_HIP(hipMemcpyAsync(sb.data(), Sjj, 8 * (size_t)mm, hipMemcpyDeviceToHost, s));
-        CUEV_HIP(hipMemcpyAsync(gb.data(), Gjj, 8 * (size_t)mm, hipMemcpyDeviceToHost, s));
+        std::vector<double> sh((size_t)mm), gh((size_t)mm), W((size_t)mm, 0.0), sb, gb, wv, work;
+        std::vector<int> iwork;
+        CUEV_HIP(hipMemcpyAsync(sh.data(), Sjj, 8 * (size_t)mm, hipMemcpyDeviceToHost, s));
+        CUEV_HIP(hipMemcpyAsync(gh.data(), Gjj, 8 * (size_t)mm, hipMemcpyDeviceToHost, s));
         CUEV_HIP(hipStreamSynchronize(s));
-        const int itype = 1;
-        const char jobz = 'V', uplo = 'L';
-        int nn = m, lda = m, info = 0;
-        int lwork = 1 + 6 * m + 2 * m * m, liwork = 3 + 5 * m;
-        std::vector<double> work(lwork);
-        std::vector<int> iwork(liwork);
-        L->dsygvd(&itype, &jobz, &uplo, &nn, sb.data(), &lda, gb.data(), &lda, wv.data(), work.data(), &lwork,
-                  iwork.data(), &liwork, &info);
-        if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "mixed: dsygvd on the clustered columns returned %d", info);
-        // W (in sb) into mb.G, X_J W into F, scattered back into X
-        CUEV_HIP(hipMemcpyAsync(mb.G, sb.data(), 8 * (size_t)mm, hipMemcpyHostToDevice, s));
+        for (size_t c = 0; c + 1 < start.size(); ++c) {
+            const int a = start[c], sz = start[c + 1] - start[c];
+            sb.assign((size_t)sz * sz, 0.0);
+            gb.assign((size_t)sz * sz, 0.0);
+            for (int q = 0; q < sz; ++q)
+                for (int r = q; r < sz; ++r) {
+                    sb[r + (size_t)q * sz] = sh[(a + r) + (size_t)(a + q) * m];
+                    gb[r + (size_t)q * sz] = gh[(a + r) + (size_t)(a + q) * m];
+                }
+            wv.assign(sz, 0.0);
+            const int itype = 1;
+            const char jobz = 'V', uplo = 'L';
+            int nn = sz, lda = sz, info = 0;
+            int lwork = 1 + 6 * sz + 2 * sz * sz, liwork = 3 + 5 * sz;
+            work.resize(lwork);
+            iwork.resize(liwork);
+            L->dsygvd(&itype, &jobz, &uplo, &nn, sb.data(), &lda, gb.data(), &lda, wv.data(), work.data(), &lwork,
+                      iwork.data(), &liwork, &info);
+            if (info != 0) CUEV_FAIL(CUEV_STATUS_LAPACK_ERROR, "mixed: dsygvd on a cluster set returned %d", info);
+            for (int q = 0; q < sz; ++q)
+                for (int r = 0; r < sz; ++r) W[(a + r) + (size_t)(a + q) * m] = sb[r + (size_t)q * sz];
+        }
+        // W into mb.G, X_J W into F, scattered back into X
+        CUEV_HIP(hipMemcpyAsync(mb.G, W.data(), 8 * (size_t)mm, hipMemcpyHostToDevice, s));
     } else {
         const unsigned gsub = (unsigned)ceil_div(mm, 256), mt = (unsigned)ceil_div(m, 64);
         // S_JJ and E = G_JJ - I in full; E S_JJ into mb.S (X_J is gathered again below);
@@ -529,6 +551,27 @@ int rayleigh_ritz(MixedBuffers &mb, hipStream_t s, const double *A, double *X, d
     hipLaunchKernelGGL(scatter_cols_kernel, dim3(gcol), dim3(256), 0, s, F, X, (i64)n, n, mb.perm, m);
     CUEV_LAUNCH_CHECK();
     CUEV_HIP(hipStreamSynchronize(s));
+    return CUEV_STATUS_SUCCESS;
+}
+
+// The Rayleigh-Ritz steps of a list of cluster sets: each set of kRRDeviceMin columns
+// or more on the GPU on its own, the smaller ones together (one pass of products over
+// their union, a host solve per set); cols receives the number of columns treated.
+int rayleigh_ritz_sets(MixedBuffers &mb, hipStream_t s, const double *A, double *X, double *F, int n,
+                       const std::vector<std::vector<int>> &sets, Scratch gs, int &cols) {
+    std::vector<int> small, start{0};
+    cols = 0;
+    for (const auto &J : sets) {
+        cols += (int)J.size();
+        if ((int)J.size() >= kRRDeviceMin) {
+            const std::vector<int> one{0, (int)J.size()};
+            CUEV_TRY(rayleigh_ritz(mb, s, A, X, F, n, J, one, true, gs));
+        } else {
+            small.insert(small.end(), J.begin(), J.end());
+            start.push_back((int)small.size());
+        }
+    }
+    if (!small.empty()) CUEV_TRY(rayleigh_ritz(mb, s, A, X, F, n, small, start, false, gs));
     return CUEV_STATUS_SUCCESS;
 }
 
@@ -631,16 +674,18 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
     // limit applies to the Newton steps
     int sweeps = 0, oa_sweeps = 0, nclus_cols = 0;
     bool did_rr = false, probe = false;
-    std::vector<int> rr_cols; // columns of the Rayleigh-Ritz step due right after the next Newton step
+    // cluster sets of the Rayleigh-Ritz steps due right after the next Newton step
+    std::vector<std::vector<int>> rr_sets;
     double res = 0, orth = 0;
     unsigned nclus = 0;
     const auto ref0 = clk::now();
     for (;;) {
         double *other = (X == evec) ? mb.T1 : evec;
-        if (!rr_cols.empty()) {
-            CUEV_TRY(rayleigh_ritz(mb, s, A, X, other, n, rr_cols, gs64));
-            nclus_cols = std::max(nclus_cols, (int)rr_cols.size());
-            rr_cols.clear();
+        if (!rr_sets.empty()) {
+            int cols = 0;
+            CUEV_TRY(rayleigh_ritz_sets(mb, s, A, X, other, n, rr_sets, gs64, cols));
+            nclus_cols = std::max(nclus_cols, cols);
+            rr_sets.clear();
             did_rr = true;
             ++sweeps;
             continue;
@@ -708,24 +753,40 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         // later, after a Newton step, from the evaluation that follows it. S and G
         // are recomputed for the rotated basis.
         if (!did_rr) {
-            CUEV_TRY(fill_int(s, mb.perm, n, 0));
-            hipLaunchKernelGGL(oa_flag_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
-                               mb.S, mb.G, (i64)n, n, mb.lam, nu, mb.perm);
-            CUEV_LAUNCH_CHECK();
+            // ranks of lam; each clustered column's partner ranks
             std::vector<double> hl(n);
-            std::vector<int> flag(n);
             CUEV_HIP(hipMemcpyAsync(hl.data(), mb.lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
-            CUEV_HIP(hipMemcpyAsync(flag.data(), mb.perm, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
             CUEV_HIP(hipStreamSynchronize(s));
-            std::vector<int> idx;
-            for (int i = 0; i < n; ++i)
-                if (flag[i]) idx.push_back(i);
-            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hl[a] < hl[b]; });
+            std::vector<int> ord(n), rank(n), hi(n);
+            std::iota(ord.begin(), ord.end(), 0);
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hl[a] < hl[b]; });
+            for (int p = 0; p < n; ++p) rank[ord[p]] = p;
+            CUEV_HIP(hipMemcpyAsync(mb.rank, rank.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
+            CUEV_TRY(fill_int(s, mb.hi, n, -1));
+            hipLaunchKernelGGL(oa_flag_kernel, dim3((unsigned)ceil_div((i64)n * n, 256)), dim3(256), 0, s,
+                               mb.S, mb.G, (i64)n, n, mb.lam, nu, mb.rank, mb.hi);
+            CUEV_LAUNCH_CHECK();
+            CUEV_HIP(hipMemcpyAsync(hi.data(), mb.hi, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+            CUEV_HIP(hipStreamSynchronize(s));
+            // Cluster sets: the rank intervals [rank, max(rank, hi)] of the clustered
+            // columns, merged where they overlap (a pair is seen from both ends, so an
+            // earlier partner's interval already reaches a column). No clustered pair
+            // joins two sets, so each takes a Rayleigh-Ritz step of its own.
+            std::vector<std::vector<int>> sets;
+            int end = -1;
+            for (int p = 0; p < n; ++p) {
+                const int i = ord[p];
+                if (hi[i] < 0) continue;
+                if (p > end) sets.emplace_back();
+                sets.back().push_back(i);
+                end = std::max(end, std::max(p, hi[i]));
+            }
             if (oa_sweeps == 0) {
-                rr_cols = std::move(idx);
-            } else if (!idx.empty()) {
-                CUEV_TRY(rayleigh_ritz(mb, s, A, X, other, n, idx, gs64));
-                nclus_cols = std::max(nclus_cols, (int)idx.size());
+                rr_sets = std::move(sets);
+            } else if (!sets.empty()) {
+                int cols = 0;
+                CUEV_TRY(rayleigh_ritz_sets(mb, s, A, X, other, n, sets, gs64, cols));
+                nclus_cols = std::max(nclus_cols, cols);
                 did_rr = true;
                 ++sweeps;
                 continue;
@@ -744,7 +805,7 @@ int run_mixed(Context<float> &cf, MixedBuffers &mb, double *A, int n, double *ev
         X = other;
         ++sweeps;
         ++oa_sweeps;
-        probe = rr_cols.empty(); // no check when the Rayleigh-Ritz step follows
+        probe = rr_sets.empty(); // no check when the Rayleigh-Ritz step follows
     }
     const auto ref1 = clk::now();
     // eigenvalues ascending; columns follow
