@@ -25,16 +25,19 @@
 //      after the next step. On a geometric spectrum the global bound (with its
 //      2-norms bounded by Frobenius norms) marks nearly every pair of an FP32 start
 //      as clustered; the pair test marks those whose coupling is comparable to
-//      their gap. Each sweep first measures the residual
+//      their gap. Each evaluation measures the residual
 //      ||A X - X diag(lam)||_F / (n ||A||_F) and ||X^T X - I||_F / n from the
-//      products it needs anyway and stops once both are below the tolerance.
-//      After the first Newton step (which leaves clustered pairs orthogonalised
-//      but not rotated), one Rayleigh-Ritz step on the columns of the clustered
-//      pairs resolves them (the cluster treatment of Ogita and Aishima's
-//      refinement for clustered eigenvalues, JJIAM 36, 2019, applied to the union
-//      of the clusters), and the next Newton step finishes the separated pairs.
-//   All products run on the FP64 MFMA engine; the fused element-wise kernels
-//   below build lam, E and the norms.
+//      products it needs anyway; after a Newton step an estimate of both from 16
+//      Gaussian probes comes first and ends the refinement when both are well
+//      below the tolerance. The columns of the pairs that the first Newton step
+//      leaves clustered (orthogonalised, not rotated) split into sets that no
+//      clustered pair joins; right after that step one Rayleigh-Ritz step per set
+//      resolves them (the cluster treatment of Ogita and Aishima's refinement for
+//      clustered eigenvalues, JJIAM 36, 2019), and the next Newton step finishes
+//      the separated pairs.
+//   A is first filled in full from its lower triangle (the solve overwrites A),
+//   so every product with it is a plain one. All products run on the FP64 MFMA
+//   engine; the fused element-wise kernels below build lam, E and the norms.
 #include "refine.hpp"
 
 #include <algorithm>
