@@ -137,6 +137,22 @@ def test_lower_triangle_only_torch_layout(torch_dev):
     check(A, lam2.cpu().numpy(), V2.cpu().numpy())
 
 
+@pytest.mark.parametrize("n,prec", [(700, "d"), (2100, "d"), (700, "s")])
+def test_lower_triangle_only_nan_above(torch_dev, n, prec):
+    """NaN above the diagonal: a read of the upper triangle that is multiplied by zero
+    (which finite garbage would hide) shows as a non-finite result. n=700 takes the
+    direct back-transform and the host D&C, n=2100 the reordered one and the device D&C."""
+    rng = np.random.default_rng(n + 31)
+    G = rng.standard_normal((n, n))
+    A = (G + G.T) / 2
+    An = np.tril(A) + np.triu(np.full((n, n), np.nan), 1)
+    lam, V = solve_np(torch_dev, An, prec=prec)
+    if prec == "d":
+        check(A, lam, V)
+    else:
+        check(A, lam, V, ev_tol=1e-5, res_tol=1e-7, orth_tol=1e-7)
+
+
 def test_invalid_arguments(torch_dev):
     torch = torch_dev
     L = cv.lib()
