@@ -531,8 +531,10 @@ template <typename T, int B> int launch_chase(Workspace<T> &ws, T *Bp) {
     // a sweep of H hops starts about two hops after the previous one, so up to
     // ~H/2 <= n/(2b) sweeps could be active at once; the grid stays a little below
     // that, leaving CUs to the back-transform GEMMs that run beside the chase
-    // (measured at n = 16384, b = 64: 112 workgroups 1.19 s per solve, 136: 1.215 s,
-    // 84: 1.215 s)
+    // (measured at n = 16384, b = 64, with the same factor in the phases below:
+    // 84 workgroups, chase 307 ms and 0.966 s per solve; 112, 276 ms and 0.931 s;
+    // 136, 278 ms and 0.935 s; 168, 274 ms and 0.931 s, with Q_s^T beside the chase
+    // going from 222 to 252 ms)
     const int useful = (int)ceil_div((i64)n * 13, 32 * B) + 8;
     const int grid = std::max(1, std::min({nsweeps, resident, useful}));
     // phases of equal sweep counts; phase p keeps the workgroups its sweeps' lengths
