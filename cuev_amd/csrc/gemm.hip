@@ -423,9 +423,11 @@ __device__ __forceinline__ void dma_tile(bool kc, const double *p, i64 ld, i64 r
 
 // Per-element load of an R x 16 tile (rows < rlim, k < klim; zero elsewhere) into the
 // DMA layouts (synchronous: used for edge tiles, K tails and unaligned operands).
+// sym: p holds the lower triangle of a symmetric matrix, (r, k) read at p[r + k*ld]
+// if r >= k, else p[k + r*ld], into either layout.
 template <int R, int NT>
 __device__ __forceinline__ void scalar_tile(bool kc, const double *p, i64 ld, i64 r0, i64 k0, i64 rlim, i64 klim,
-                                            char *s, int tid) {
+                                            char *s, int tid, bool sym = false) {
     double *d = reinterpret_cast<double *>(s);
     constexpr int PER = R * 16 / NT;
     // one element at a time (edge tiles only): few live registers beside the accumulators
@@ -441,7 +443,8 @@ __device__ __forceinline__ void scalar_tile(bool kc, const double *p, i64 ld, i6
             kk = idx / R;
         }
         const i64 r = r0 + row, k = k0 + kk;
-        const double v = (r < rlim && k < klim) ? (kc ? p[k + r * ld] : p[r + k * ld]) : 0.0;
+        const i64 off = sym ? ((r >= k) ? r + k * ld : k + r * ld) : (kc ? k + r * ld : r + k * ld);
+        const double v = (r < rlim && k < klim) ? p[off] : 0.0;
         if (kc) d[(row * 8 + ((kk >> 1) ^ kc_swz(row))) * 2 + (kk & 1)] = v;
         else d[kk * DmaTile<R>::RSTR + row] = v;
     }
@@ -469,7 +472,13 @@ __device__ __forceinline__ void gemm_body_dma(const KParams<double> &p) {
     constexpr int TM = BM / WM, TN = BN / WN;
     constexpr int FM = TM / 16, FN = TN / 16;
     constexpr bool AKC = AMODE == A_T, BKC = BMODE == B_N;
-    constexpr int ABYTES = DmaTile<BM>::template bytes<AKC>();
+    // A_SYM: a k-tile below the diagonal is read row-contiguous, one above it
+    // k-contiguous (the transpose of stored lower entries), one across it per element
+    constexpr bool ASYM = AMODE == A_SYM;
+    constexpr int ABYTES = ASYM ? (DmaTile<BM>::template bytes<true>() > DmaTile<BM>::template bytes<false>()
+                                       ? DmaTile<BM>::template bytes<true>()
+                                       : DmaTile<BM>::template bytes<false>())
+                                : DmaTile<BM>::template bytes<AKC>();
     constexpr int BBYTES = DmaTile<BN>::template bytes<BKC>();
     constexpr int STAGE = ABYTES + BBYTES;
     using V4 = Mfma<double>::v4;
@@ -512,8 +521,14 @@ __device__ __forceinline__ void gemm_body_dma(const KParams<double> &p) {
                     ko = p.asplit;
                 }
             }
-            if (m_in && k_in && p.avec) dma_tile<BM, NT>(AKC, pa, lda, m0, k0 - ko, st, tid);
-            else scalar_tile<BM, NT>(AKC, pa, lda, m0, k0 - ko, p.M, kend - ko, st, tid);
+            if constexpr (ASYM) {
+                const bool above = m0 + BM - 1 <= k0, below = m0 >= k0 + BK - 1;
+                if ((above || below) && m_in && k_in && p.avec) dma_tile<BM, NT>(above, pa, lda, m0, k0, st, tid);
+                else scalar_tile<BM, NT>(above, pa, lda, m0, k0, p.M, kend, st, tid, true);
+            } else {
+                if (m_in && k_in && p.avec) dma_tile<BM, NT>(AKC, pa, lda, m0, k0 - ko, st, tid);
+                else scalar_tile<BM, NT>(AKC, pa, lda, m0, k0 - ko, p.M, kend - ko, st, tid);
+            }
         }
         {
             const double *pb = p.b0;
@@ -539,11 +554,12 @@ __device__ __forceinline__ void gemm_body_dma(const KParams<double> &p) {
         __syncthreads();
         if (kt + 1 < nkt) issue(kt + 1);
         const char *st = smem + (kt & 1) * STAGE;
+        const bool akc = ASYM ? (m0 + BM - 1 <= kbeg + (i64)kt * BK) : AKC;
 #pragma unroll
         for (int kg = 0; kg < BK / 8; ++kg) {
             vec af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = dma_frag<BM>(AKC, st, wm * TM + i * 16, kg, li, lj);
+            for (int i = 0; i < FM; ++i) af[i] = dma_frag<BM>(akc, st, wm * TM + i * 16, kg, li, lj);
 #pragma unroll
             for (int j = 0; j < FN; ++j) bf[j] = dma_frag<BN>(BKC, st + ABYTES, wn * TN + j * 16, kg, li, lj);
 #pragma unroll
@@ -588,17 +604,19 @@ __device__ __forceinline__ void gemm_body_dma(const KParams<double> &p) {
 }
 
 // The DMA path serves FP64 tiles of at least 128 x 128 with plain or K-concatenated
-// operands when both operands allow 16-byte transfers; symmetric A, unaligned
-// operands and everything else take the register-staged path, which overlaps its
-// per-element loads with the computation.
-template <typename T, int BM, int BN, int AMODE>
+// operands, and symmetric A in 128 x 64 tiles (A22 W of the band reduction, W's tile
+// k-contiguous), when both operands allow 16-byte transfers; unaligned operands and
+// everything else take the register-staged path, which overlaps its per-element
+// loads with the computation.
+template <typename T, int BM, int BN, int AMODE, int BMODE>
 constexpr bool dma_capable() {
-    return sizeof(T) == 8 && BM >= 128 && BN >= 128 && AMODE != A_SYM;
+    if constexpr (AMODE == A_SYM) return sizeof(T) == 8 && BM == 128 && BN == 64 && BMODE == B_N;
+    else return sizeof(T) == 8 && BM >= 128 && BN >= 128;
 }
 
 template <typename T, int BM, int BN, int WM, int WN, int AMODE, int BMODE, int MINB, bool DMA>
 __global__ __launch_bounds__(64 * WM * WN, MINB) void gemm_tile_kernel(const KParams<T> p) {
-    if constexpr (DMA && dma_capable<T, BM, BN, AMODE>()) {
+    if constexpr (DMA && dma_capable<T, BM, BN, AMODE, BMODE>()) {
         gemm_body_dma<BM, BN, WM, WN, AMODE, BMODE>(p);
     } else {
         gemm_body_regs<T, BM, BN, WM, WN, AMODE, BMODE>(p);
@@ -671,12 +689,13 @@ __global__ __launch_bounds__(64 * kWideWaves) void splitk_reduce_wide_kernel(con
 }
 
 // Tile configurations: (BM, BN, WM, WN, blocks per CU the launch bounds ask for).
-enum Cfg { C128x128 = 0, C128x64 = 1, C64x128 = 2, C64x64 = 3, C256x128 = 4 };
+// C128x64W8: 128 x 64 on eight waves, the FP64 symmetric panel product's LDS-DMA tiles.
+enum Cfg { C128x128 = 0, C128x64 = 1, C64x128 = 2, C64x64 = 3, C256x128 = 4, C128x64W8 = 5 };
 
 template <typename T, int BM, int BN, int WM, int WN, int MINB, int AM, int BMD>
 int launch_one(hipStream_t s, const KParams<T> &p, int splits, int ntiles) {
     dim3 grid((unsigned)ntiles, (unsigned)splits);
-    if constexpr (dma_capable<T, BM, BN, AM>()) {
+    if constexpr (dma_capable<T, BM, BN, AM, BMD>()) {
         if (p.avec && p.bvec) {
             hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, WM, WN, AM, BMD, MINB, true>), grid, dim3(64 * WM * WN),
                                0, s, p);
@@ -697,6 +716,9 @@ int launch_cfg(hipStream_t s, const KParams<T> &p, int splits, int ntiles, int c
     case C128x64: return launch_one<T, 128, 64, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
     case C64x128: return launch_one<T, 64, 128, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
     case C256x128: return launch_one<T, 256, 128, 4, 2, 1, AM, BMD>(s, p, splits, ntiles);
+    case C128x64W8:
+        if constexpr (sizeof(T) == 8) return launch_one<T, 128, 64, 4, 2, 1, AM, BMD>(s, p, splits, ntiles);
+        else return launch_one<T, 64, 64, 2, 2, 2, AM, BMD>(s, p, splits, ntiles); // not selected for FP32
     default: return launch_one<T, 64, 64, 2, 2, 2, AM, BMD>(s, p, splits, ntiles);
     }
 }
@@ -714,7 +736,7 @@ int launch_modes(hipStream_t s, const KParams<T> &p, int splits, int ntiles, int
     CUEV_FAIL(CUEV_STATUS_NOT_SUPPORTED, "gemm: operand mode combination (%d, %d) not implemented", am, bm);
 }
 
-int cfg_bm(int cfg) { return (cfg == C128x128 || cfg == C128x64) ? 128 : (cfg == C256x128) ? 256 : 64; }
+int cfg_bm(int cfg) { return (cfg == C128x128 || cfg == C128x64 || cfg == C128x64W8) ? 128 : (cfg == C256x128) ? 256 : 64; }
 int cfg_bn(int cfg) { return (cfg == C128x128 || cfg == C64x128 || cfg == C256x128) ? 128 : 64; }
 
 template <typename T> bool vec_ok(const T *p, i64 ld) {
@@ -787,9 +809,10 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     } else if (lower) {
         cfg = (N >= 1024) ? C128x128 : C64x64;
     } else if (sym_panel) {
-        // symmetric A times a panel-wide B (the band reduction's A22 W): bandwidth-bound
-        // on A, so small tiles and deep split-K keep more loads in flight
-        cfg = C64x64;
+        // symmetric A times a panel-wide B (the band reduction's A22 W): FP64 on the
+        // LDS-DMA 128 x 64 tiles with eight waves (four waves reach two per SIMD only),
+        // FP32 on small register-staged tiles; deep split-K
+        cfg = (sizeof(T) == 8 && !B.trans && B.split < 0) ? C128x64W8 : C64x64;
     } else {
         const int bm = (M > 64) ? 128 : 64, bn = (N > 64) ? 128 : 64;
         cfg = (bm == 128 && bn == 128) ? C128x128 : (bm == 128) ? C128x64 : (bn == 128) ? C64x128 : C64x64;
@@ -810,7 +833,7 @@ int gemm_ex(hipStream_t s, i64 M, i64 N, i64 K, T alpha, const Operand<T> &A,
     // loads in flight than a few long ones. Tall-skinny updates (N <= 64, short K) are
     // split four ways.
     int splits = 1;
-    const i64 target = (sym_panel ? 16LL : 2LL) * cus;
+    const i64 target = (sym_panel ? (cfg == C128x64W8 ? 8LL : 16LL) : 2LL) * cus;
     const i64 kmin = sym_panel ? 256 : 64; // shortest K chunk per split
     if (K == 0) {
         splits = 0;
