@@ -406,7 +406,9 @@ __device__ __forceinline__ void dma_tile(bool kc, const double *p, i64 ld, i64 r
             const double *g = p + (r0 + row) * ld + k0 + 2 * c;
             __builtin_amdgcn_global_load_lds((gvoid_t *)g, (lvoid_t *)(s + inst * 1024), 16, 0, 0);
         }
-    } else {
+    } else if constexpr (R >= 128) {
+        // (row-contiguous tiles of fewer than 128 rows never take this path: the
+        // 64-row DMA tiles are k-contiguous B operands only)
         constexpr int PER_K = R / 128; // 1 KB instructions per k row
         constexpr int NI = 16 * PER_K / NW;
 #pragma unroll
