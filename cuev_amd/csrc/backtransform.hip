@@ -32,6 +32,8 @@
 #include "backtransform.hpp"
 
 #include <algorithm>
+#include <utility>
+#include <vector>
 
 #include "gemm.hpp"
 #include "profile.hpp"
@@ -42,6 +44,8 @@ namespace {
 
 constexpr int kBTThreads = 256; // T-factor kernel: one wave per block reflector
 constexpr int kBBThreads = 128; // apply kernel: independent waves, 2 per workgroup (LDS <= 64 KB)
+// blocks per launch of the paired apply kernel (the whole stream up to n = 16500)
+constexpr i64 kBtSegBlocks = 140000;
 
 template <typename T> struct MfmaBt;
 template <> struct MfmaBt<double> {
@@ -452,8 +456,10 @@ template <int B> __device__ __forceinline__ int bt_gmax(int n, int h) {
     return m < 0 ? -1 : m / kBtGroup;
 }
 
-template <int B, bool TR> __device__ __forceinline__ void bt_step_first(BtStep &s, int n, const i64 *__restrict__ toff) {
-    s.h0 = TR ? ((n - 2) / B) / 2 * 2 : 0;
+// the first step of the pass starting at hop h0
+template <int B, bool TR>
+__device__ __forceinline__ void bt_step_first(BtStep &s, int n, const i64 *__restrict__ toff, int h0) {
+    s.h0 = h0;
     s.gA = bt_gmax<B>(n, s.h0);
     s.gB = bt_gmax<B>(n, s.h0 + 1);
     s.g = TR ? (s.gB >= 0 ? -1 : 0) : s.gA + 1;
@@ -521,7 +527,9 @@ __device__ __forceinline__ void bt_load_half(T (&o)[2 * NP], const T *__restrict
 }
 
 // M <- Q_b M (TR: Q_b^T M) for columns [0, ncols), one wave per 16 columns, two
-// blocks per step (see BtStep): the two blocks' products are interleaved, so the
+// blocks per step (see BtStep), over the passes from the one starting at hop h_first
+// to the one starting at hop h_last (in application order; a pass leaves all its
+// rows in M, so a launch may end and the next begin at any pass): the two blocks' products are interleaved, so the
 // MFMA pipe has independent work while either block's chain drains, and a step's
 // fixed costs (LDS round trip, window bookkeeping) are shared by two blocks. The
 // window lives in a ring of 2 NKB + 1 tiles of 16 rows per wave (24 KB at b = 64,
@@ -534,7 +542,8 @@ template <typename T, int B, bool TR>
 __global__ __launch_bounds__(kBBThreads) void bc_back_pair_kernel(const T *__restrict__ ops,
                                                                   const i64 *__restrict__ toff,
                                                                   i64 nblocks, const T *__restrict__ zp,
-                                                                  T *M, i64 ldm, int n, int ncols) {
+                                                                  T *M, i64 ldm, int n, int ncols, int h_first,
+                                                                  int h_last) {
     using MB = MfmaBt<T>;
     using V4 = typename MB::v4;
     constexpr int NKB = B / 16 + 1; // tiles of a block
@@ -584,7 +593,7 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_pair_kernel(const T *__res
     } while (0)
 
     BtStep cur;
-    bt_step_first<B, TR>(cur, n, toff);
+    bt_step_first<B, TR>(cur, n, toff, h_first);
     int top;
     i64 il, iu;
     bt_step_blocks<B, TR>(cur, n, nblocks, il, iu, top);
@@ -603,7 +612,7 @@ __global__ __launch_bounds__(kBBThreads) void bc_back_pair_kernel(const T *__res
     for (;;) {
         if (putent) BP_PUT16(slot(TR ? NW - 1 : 0), ent);
         BtStep nx = cur;
-        const bool more = bt_step_next<B, TR>(nx, n, toff);
+        const bool more = bt_step_next<B, TR>(nx, n, toff) && (TR ? nx.h0 >= h_last : nx.h0 <= h_last);
         int ntop = top;
         i64 nil = il, niu = iu;
         if (more) bt_step_blocks<B, TR>(nx, n, nblocks, nil, niu, ntop);
@@ -726,18 +735,46 @@ int launch_bc_back(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, b
                                                               kBBThreads, 0));
     const i64 cols_per_block = 16 * (kBBThreads / 64);
     const i64 chunk = std::max<i64>(1, (i64)per_cu * device_cu_count()) * cols_per_block;
+    // The paired kernel's passes in launches of at least kBtSegBlocks blocks each (the
+    // last one fewer): the waves of one launch drift apart along the block stream as
+    // it goes, and a launch boundary lines them up again. Over the stream of
+    // n = 16384 (1.3e5 blocks) they stay within L2 reach of each other (hit rate
+    // 0.90), so up to there it is one launch.
+    std::vector<std::pair<int, int>> segs; // (first, last) pass start hops, in application order
+    if constexpr (B == 64) {
+        auto gmax = [&](int h) {
+            const int m = std::min(n - 3, n - 2 - h * B);
+            return m < 0 ? -1 : m / kBtGroup;
+        };
+        std::vector<int> passes;
+        if (tr) {
+            for (int h0 = ((n - 2) / B) / 2 * 2; h0 >= 0; h0 -= 2) passes.push_back(h0);
+        } else {
+            for (int h0 = 0; gmax(h0) >= 0; h0 += 2) passes.push_back(h0);
+        }
+        i64 acc = 0;
+        for (size_t q = 0; q < passes.size(); ++q) {
+            if (acc == 0) segs.emplace_back(passes[q], passes[q]);
+            segs.back().second = passes[q];
+            acc += (i64)(gmax(passes[q]) + 1) + (i64)(gmax(passes[q] + 1) + 1);
+            if (acc >= kBtSegBlocks) acc = 0;
+        }
+    }
     for (i64 c0 = 0; c0 < n; c0 += chunk) {
         const int nc = (int)std::min<i64>(chunk, n - c0);
         const dim3 grid((unsigned)ceil_div(nc, cols_per_block));
         T *Mc = M + c0 * ldm;
         if constexpr (B == 64) {
             // two blocks per step (passes of two hops: the default band's window fits)
-            if (tr)
-                hipLaunchKernelGGL((bc_back_pair_kernel<T, B, true>), grid, dim3(kBBThreads), 0, s, ws.BtT,
-                                   ws.bt_off, ws.bt_blocks, zp, Mc, ldm, n, nc);
-            else
-                hipLaunchKernelGGL((bc_back_pair_kernel<T, B, false>), grid, dim3(kBBThreads), 0, s, ws.BtT,
-                                   ws.bt_off, ws.bt_blocks, zp, Mc, ldm, n, nc);
+            for (const auto &sg : segs) {
+                if (tr)
+                    hipLaunchKernelGGL((bc_back_pair_kernel<T, B, true>), grid, dim3(kBBThreads), 0, s, ws.BtT,
+                                       ws.bt_off, ws.bt_blocks, zp, Mc, ldm, n, nc, sg.first, sg.second);
+                else
+                    hipLaunchKernelGGL((bc_back_pair_kernel<T, B, false>), grid, dim3(kBBThreads), 0, s, ws.BtT,
+                                       ws.bt_off, ws.bt_blocks, zp, Mc, ldm, n, nc, sg.first, sg.second);
+                CUEV_LAUNCH_CHECK();
+            }
         } else {
             if (tr)
                 hipLaunchKernelGGL((bc_back_kernel<T, B, true>), grid, dim3(kBBThreads), 0, s, ws.BtT,

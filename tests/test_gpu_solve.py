@@ -250,6 +250,26 @@ def test_reordered_repeatable_n16384(torch_dev):
             assert torch.equal(lam, first)
 
 
+@pytest.mark.parametrize("bt", ["reordered", "direct"])
+def test_bc_back_in_segments_n18000(torch_dev, bt):
+    """n=18000: the bc_back stream (1.6e5 block reflectors) is applied in two launches
+    of pass ranges (each launch lines its waves up again) and in two column chunks, in
+    both back-transform orders (Q_b^T on M, Q_b on Q_d). Known spectrum; residual and
+    orthogonality at the north-star tolerances, computed on the GPU in float64."""
+    torch = torch_dev
+    from cuev_amd import workloads
+    n = 18000
+    A0, lam_exact = workloads.make("q16k", n=n, device="cuda")
+    lam, V = cv.symm_eig_solve(A0, bandwidth=64, block=512, bt_mode=bt, dc_mode="device")
+    torch.cuda.synchronize()
+    ref = torch.sort(lam_exact.double()).values
+    assert (torch.sort(lam.double()).values - ref).abs().max().item() <= 1e-12 * ref.abs().max().item()
+    res, orth = workloads.residual_orth(A0, V, lam.double())
+    assert res <= 1e-13 and orth <= 1e-13, (res, orth)
+    assert res * n <= 1e-13, ("bench_solver-form residual", res * n)
+    del A0, V, lam
+
+
 def test_reordered_above_one_round_n16500(torch_dev):
     """n=16500 is more columns than one resident round of the bc_back kernel covers
     (16384 at one wave per SIMD), so Q_b^T is applied in two column chunks. Known
