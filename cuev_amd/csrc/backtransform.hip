@@ -44,8 +44,8 @@ namespace {
 
 constexpr int kBTThreads = 256; // T-factor kernel: one wave per block reflector
 constexpr int kBBThreads = 128; // apply kernel: independent waves, 2 per workgroup (LDS <= 64 KB)
-// blocks per launch of the paired apply kernel (the whole stream up to n = 16500)
-constexpr i64 kBtSegBlocks = 140000;
+// blocks per launch of the paired apply kernel (a quarter of the stream at n = 16384)
+constexpr i64 kBtSegBlocks = 33000;
 
 template <typename T> struct MfmaBt;
 template <> struct MfmaBt<double> {
@@ -737,9 +737,10 @@ int launch_bc_back(Workspace<T> &ws, hipStream_t s, const T *U, T *M, i64 ldm, b
     const i64 chunk = std::max<i64>(1, (i64)per_cu * device_cu_count()) * cols_per_block;
     // The paired kernel's passes in launches of at least kBtSegBlocks blocks each (the
     // last one fewer): the waves of one launch drift apart along the block stream as
-    // it goes, and a launch boundary lines them up again. Over the stream of
-    // n = 16384 (1.3e5 blocks) they stay within L2 reach of each other (hit rate
-    // 0.90), so up to there it is one launch.
+    // it goes, and a launch boundary lines them up again (measured at n = 16384,
+    // 1.3e5 blocks: one launch 252-260 ms, two 245 ms, four 243.6 ms, eight 244.7 ms,
+    // sixteen 245.3 ms; at n = 49152, 1.18e6 blocks per column chunk: one launch
+    // 7.6-12 s, nine 6.6 s).
     std::vector<std::pair<int, int>> segs; // (first, last) pass start hops, in application order
     if constexpr (B == 64) {
         auto gmax = [&](int h) {

@@ -252,10 +252,10 @@ def test_reordered_repeatable_n16384(torch_dev):
 
 @pytest.mark.parametrize("bt", ["reordered", "direct"])
 def test_bc_back_in_segments_n18000(torch_dev, bt):
-    """n=18000: the bc_back stream (1.6e5 block reflectors) is applied in two launches
-    of pass ranges (each launch lines its waves up again) and in two column chunks, in
-    both back-transform orders (Q_b^T on M, Q_b on Q_d). Known spectrum; residual and
-    orthogonality at the north-star tolerances, computed on the GPU in float64."""
+    """n=18000: the bc_back stream (1.6e5 block reflectors) is applied in launches of
+    pass ranges (five; each launch lines its waves up again) and in two column chunks,
+    in both back-transform orders (Q_b^T on M, Q_b on Q_d). Known spectrum; residual
+    and orthogonality at the north-star tolerances, computed on the GPU in float64."""
     torch = torch_dev
     from cuev_amd import workloads
     n = 18000
